@@ -1,0 +1,74 @@
+"""Builds the gfx950 library hyperflow_amd/libhyperflow_amd.so with hipcc.
+
+Kernels are compiled per arithmetic type: the FP64 unit with
+-ffp-contract=off (no fused multiply-add, the reference's rounding), the FP32
+unit with contraction allowed.  Packed-FP32 instructions are disabled for all
+device code.  The host C ABI is compiled with -ffp-contract=off as well, since
+it evaluates FP64 timestep arithmetic on the CPU.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "libhyperflow_amd.so")
+ARCH = os.environ.get("HF_OFFLOAD_ARCH", "gfx950")
+
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+COMMON = ["-std=c++17", "-fPIC", "-fvisibility=hidden", f"-I{os.path.join(ROOT, 'include')}", f"-I{CSRC}"]
+DEVICE = [f"--offload-arch={ARCH}", "-O3", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+UNITS = [
+    ("hf_kernels_f64.hip", DEVICE + ["-ffp-contract=off"]),
+    ("hf_kernels_f32.hip", DEVICE + ["-ffp-contract=fast"]),
+    ("hf_capi.cpp", ["-O2", "-ffp-contract=off"]),
+]
+
+
+def _compile(unit, flags, outdir):
+    src = os.path.join(CSRC, unit)
+    obj = os.path.join(outdir, unit + ".o")
+    cmd = [HIPCC] + COMMON + flags + ["-c", src, "-o", obj]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    msgs = [ln for ln in r.stdout.splitlines() if "packed-fp32-ops" not in ln]
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {unit}:\n" + "\n".join(msgs))
+    return obj
+
+
+def build(verbose: bool = False) -> str:
+    outdir = os.path.join(HERE, "_build")
+    os.makedirs(outdir, exist_ok=True)
+    with ThreadPoolExecutor(max_workers=len(UNITS)) as ex:
+        objs = list(ex.map(lambda u: _compile(u[0], u[1], outdir), UNITS))
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed:\n" + r.stdout)
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f"built {LIB}")
+    return LIB
+
+
+def build_oracle(verbose: bool = False) -> None:
+    """Compile oracle/ (and oracle/_ref when the reference checkout exists)."""
+    r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "-j8"], stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stdout)
+    if verbose and r.stdout.strip():
+        print(r.stdout)
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    if "--oracle" in sys.argv:
+        build_oracle(verbose=True)

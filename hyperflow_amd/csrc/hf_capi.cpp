@@ -1,0 +1,915 @@
+// hf_capi.cpp -- the C ABI (include/hyperflow.h) over the gfx950 kernels.
+//
+// A context owns two state buffers (ping-pong).  buf[cur] always holds the
+// complete current state (interior and ghosts).  A step fills the ghosts of
+// buf[cur^1] from buf[cur] (table-driven gather, the reference's side order
+// replayed on the host), then the update kernels read the interior from
+// buf[cur] and the ghosts from buf[cur^1] and write the new interior into
+// buf[cur^1]; cur flips.  The dt of every step stays on the device: the
+// update kernels derive it from per-block wave-speed maxima that the previous
+// update and the ghost fill reduced, and from a device-resident TimeStep call
+// counter that the ghost fill advances.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/hyperflow.h"
+#include "hf_common.h"
+#include "hf_launch.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HF_HIP(call)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(HF_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+inline double smin(double a, double b) { return (b < a) ? b : a; }
+inline double smax(double a, double b) { return (a < b) ? b : a; }
+inline double bitsd(unsigned long long b)
+{
+    double d;
+    std::memcpy(&d, &b, sizeof d);
+    return d;
+}
+
+// Symbolic value of a cell during the host replay of Mesh::apply_bcs: the
+// pre-fill cell (or Dirichlet constant) it holds, and momentum negations.
+struct Prov {
+    int32_t blk;    // source block, or -1 for a constant
+    int32_t x, y;   // source cell
+    int32_t cidx;   // constant index (block*4 + side)
+    uint32_t flip;  // HF_FLIP_U | HF_FLIP_V
+};
+
+}  // namespace
+
+struct hf_ctx {
+    int scheme = 0, limiter = 0, prec = 0, dt_mode = 0, g = 1, nb = 0, np = 0;
+    double gamma = 1.4, cfl = 0.5;
+    size_t esize = 8;
+    std::vector<hf_block_desc> bdesc;
+    std::vector<HfBlockDev> blocks;
+    std::vector<HfProbDev> probs;
+    std::vector<HfItem> items;
+    std::vector<int> item_begin, item_count;      // per block
+    std::vector<std::pair<int, int>> rounds;      // item ranges launched together
+    int64_t total = 0;                            // elements per state buffer
+    void* buf[2] = {nullptr, nullptr};
+    int cur = 0;
+    bool ghosts_pending = false;   // buf[cur^1]'s ghosts equal buf[cur]'s (apply_bcs done)
+    bool partials_valid = false;   // imax[cur] describes buf[cur], gmax[cur] is zero
+
+    HfBlockDev* d_blocks = nullptr;
+    HfProbDev* d_probs = nullptr;
+    HfItem* d_items = nullptr;
+    HfGhost* d_fill = nullptr;     // ghost-fill table
+    HfGhost* d_copy = nullptr;     // ghost identity table (copy ghosts between buffers)
+    int nfill = 0, ncopy = 0;
+    void* d_consts = nullptr;
+    unsigned long long* d_imax = nullptr;   // [2][nb][2]
+    unsigned long long* d_gmax = nullptr;   // [2][nb][2]
+    unsigned long long* d_scan = nullptr;   // [nb][2] scan results + [nb][2] scratch maxima
+    uint32_t* d_calls = nullptr;
+    uint32_t* h_calls_pinned = nullptr;
+    hipEvent_t calls_copied = nullptr;   // the last upload of h_calls_pinned has been read
+    double* d_hist = nullptr;
+    int hist_cap = 1 << 20;
+    double* d_part = nullptr;
+    int part_wg = 256;
+    void* d_resid = nullptr;
+    int64_t resid_elems = 0;
+    std::vector<int64_t> calls;     // host mirror of the TimeStep call counters
+    hipStream_t stream = nullptr;
+    std::map<std::pair<int, int>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
+
+    bool profile = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending, ev_free;
+    int64_t prof_launches = 0;
+    double prof_ms = 0.0;
+
+    unsigned long long* imax(int parity) { return d_imax + (size_t)parity * nb * 2; }
+    unsigned long long* gmax(int parity) { return d_gmax + (size_t)parity * nb * 2; }
+};
+
+namespace {
+
+// ------------------------------------------------------------ validation --
+int validate(const hf_case_desc* d)
+{
+    if (!d || !d->blocks || d->nblocks < 1) return fail(HF_EINVAL, "case: no blocks");
+    if (d->scheme != HF_GODUNOV && d->scheme != HF_MUSCL_HANCOCK) return fail(HF_EINVAL, "case: unknown scheme");
+    if (d->scheme == HF_MUSCL_HANCOCK && d->limiter != HF_MINBEE && d->limiter != HF_VANLEER)
+        return fail(HF_EINVAL, "case: unknown limiter");
+    if (d->precision != HF_FP64 && d->precision != HF_FP32) return fail(HF_EINVAL, "case: unknown precision");
+    if (d->dt_mode != HF_DT_COMPAT && d->dt_mode != HF_DT_UNIFORM) return fail(HF_EINVAL, "case: unknown dt mode");
+    if (!std::isfinite(d->gamma) || !std::isfinite(d->cfl) || d->cfl <= 0.0)
+        return fail(HF_EINVAL, "case: gamma and cfl must be finite, cfl > 0");
+    if (d->tile_rows < 0) return fail(HF_EINVAL, "case: tile_rows must be >= 0");
+    const int g = d->scheme == HF_MUSCL_HANCOCK ? 2 : 1;
+    int prev_problem = 0;
+    for (int b = 0; b < d->nblocks; b++) {
+        const hf_block_desc& B = d->blocks[b];
+        char where[64];
+        std::snprintf(where, sizeof where, "block %d: ", b);
+        if (B.nx < 1 || B.ny < 1) return fail(HF_EINVAL, std::string(where) + "needs nx, ny >= 1");
+        if (!(B.x_right > B.x_left) || !(B.y_top > B.y_bottom) || !std::isfinite(B.x_right - B.x_left) ||
+            !std::isfinite(B.y_top - B.y_bottom))
+            return fail(HF_EINVAL, std::string(where) + "extents must be finite with right > left, top > bottom");
+        const int64_t tx = B.nx + 2 * g, ty = B.ny + 2 * g;
+        if ((tx + 64) * ty >= (int64_t)HF_CELL_MASK) return fail(HF_EINVAL, std::string(where) + "too many cells");
+        if (B.problem < prev_problem || B.problem > prev_problem + 1 || (b == 0 && B.problem != 0))
+            return fail(HF_EINVAL, std::string(where) + "problems must be numbered 0,1,.. over contiguous blocks");
+        prev_problem = B.problem;
+        for (int s = 0; s < 4; s++) {
+            const int k = B.bc[s];
+            if (k < HF_TRANSMISSIVE || k > HF_INTERNAL) return fail(HF_EINVAL, std::string(where) + "unknown bc");
+            if (k == HF_INTERNAL) {
+                const int n = B.neighbour[s];
+                if (n < 0 || n >= d->nblocks) return fail(HF_EINVAL, std::string(where) + "Internal neighbour out of range");
+                const hf_block_desc& E = d->blocks[n];
+                if (E.problem != B.problem)
+                    return fail(HF_EINVAL, std::string(where) + "Internal neighbour in another problem");
+                if ((s == HF_LEFT || s == HF_RIGHT) ? (E.ny != B.ny) : (E.nx != B.nx))
+                    return fail(HF_EINVAL, std::string(where) + "Internal neighbour must match the shared edge's cell count");
+            }
+            if (k == HF_DIRICHLET)
+                for (int v = 0; v < 4; v++)
+                    if (!std::isfinite(B.state[s][v])) return fail(HF_EINVAL, std::string(where) + "non-finite Dirichlet state");
+        }
+    }
+    return HF_OK;
+}
+
+// ------------------------------------------------ ghost table (host replay) --
+// Replays Mesh::apply_bcs (mesh.cpp:31-39) on provenance records instead of
+// values.  The side loops follow Transmissive (transmissive_boundary.cpp:25-104),
+// Reflective (reflective_boundary.cpp:25-120), Dirichlet
+// (dirichlet_boundary.cpp:25-104) and Internal (internal_boundary.cpp:27-108):
+// every side writes its g layers over the block's full padded extent.
+struct Replay {
+    const hf_ctx& c;
+    std::vector<std::vector<Prov>> prov;   // per block, ghost cells in perimeter order
+
+    explicit Replay(const hf_ctx& ctx) : c(ctx), prov(ctx.nb)
+    {
+        for (int b = 0; b < c.nb; b++) {
+            const HfBlockDev& B = c.blocks[b];
+            prov[b].resize(count(B));
+            for (int y = 0; y < B.ty; y++)
+                for (int x = 0; x < B.tx; x++)
+                    if (ghost(B, x, y)) prov[b][index(B, x, y)] = Prov{b, x, y, -1, 0u};
+        }
+    }
+    static bool ghost(const HfBlockDev& B, int x, int y)
+    {
+        return x < B.g || x >= B.tx - B.g || y < B.g || y >= B.ty - B.g;
+    }
+    static int64_t count(const HfBlockDev& B) { return 2LL * B.g * B.tx + 2LL * B.g * (B.ty - 2 * B.g); }
+    // perimeter index: bottom band, top band, then left/right of the middle rows
+    static int64_t index(const HfBlockDev& B, int x, int y)
+    {
+        const int g = B.g;
+        if (y < g) return (int64_t)y * B.tx + x;
+        if (y >= B.ty - g) return (int64_t)g * B.tx + (int64_t)(y - (B.ty - g)) * B.tx + x;
+        const int64_t base = 2LL * g * B.tx + (int64_t)(y - g) * 2 * g;
+        return x < g ? base + x : base + g + (x - (B.tx - g));
+    }
+    Prov read(int b, int x, int y) const
+    {
+        const HfBlockDev& B = c.blocks[b];
+        if (ghost(B, x, y)) return prov[b][index(B, x, y)];
+        return Prov{b, x, y, -1, 0u};
+    }
+    void write(int b, int x, int y, const Prov& p) { prov[b][index(c.blocks[b], x, y)] = p; }
+
+    void side(int b, int s)
+    {
+        const HfBlockDev& B = c.blocks[b];
+        const hf_block_desc& D = c.bdesc[b];
+        const int g = B.g, tx = B.tx, ty = B.ty, kind = D.bc[s];
+        const int e = kind == HF_INTERNAL ? D.neighbour[s] : -1;
+        const HfBlockDev* E = e >= 0 ? &c.blocks[e] : nullptr;
+        const int n = (s == HF_BOTTOM || s == HF_TOP) ? tx : ty;
+        const uint32_t negate = (s == HF_BOTTOM || s == HF_TOP) ? HF_FLIP_V : HF_FLIP_U;
+        for (int k = 0; k < n; k++) {
+            int gx[2], gy[2], sx[2], sy[2];
+            for (int l = 0; l < g; l++) {   // l = 0: layer next to the interior
+                switch (s) {
+                case HF_BOTTOM:
+                    gx[l] = k; gy[l] = g - 1 - l;
+                    sx[l] = k; sy[l] = E ? E->ty - g - 1 - l : g + l;
+                    break;
+                case HF_RIGHT:
+                    gx[l] = tx - g + l; gy[l] = k;
+                    sx[l] = E ? g + l : tx - g - 1 - l; sy[l] = k;
+                    break;
+                case HF_TOP:
+                    gx[l] = k; gy[l] = ty - g + l;
+                    sx[l] = k; sy[l] = E ? g + l : ty - g - 1 - l;
+                    break;
+                default:
+                    gx[l] = g - 1 - l; gy[l] = k;
+                    sx[l] = E ? E->tx - g - 1 - l : g + l; sy[l] = k;
+                    break;
+                }
+            }
+            // Internal right/top write the outer layer first (internal_boundary.cpp:62-63,104-105)
+            int order[2] = {0, 1};
+            if (kind == HF_INTERNAL && g == 2 && (s == HF_RIGHT || s == HF_TOP)) { order[0] = 1; order[1] = 0; }
+            for (int q = 0; q < g; q++) {
+                const int l = order[q];
+                Prov p;
+                if (kind == HF_DIRICHLET) p = Prov{-1, 0, 0, b * 4 + s, 0u};
+                else p = read(E ? e : b, sx[l], sy[l]);
+                write(b, gx[l], gy[l], p);
+            }
+            if (kind == HF_REFLECTIVE)
+                for (int q = 0; q < g; q++) {
+                    Prov p = read(b, gx[q], gy[q]);
+                    p.flip ^= negate;
+                    write(b, gx[q], gy[q], p);
+                }
+        }
+    }
+};
+
+void build_tables(const hf_ctx& c, std::vector<HfGhost>& fill, std::vector<HfGhost>& copy)
+{
+    Replay r(c);
+    for (int b = 0; b < c.nb; b++)
+        for (int s = 0; s < 4; s++) r.side(b, s);
+    for (int b = 0; b < c.nb; b++) {
+        const HfBlockDev& B = c.blocks[b];
+        // emit in memory order of the destination for coalescing
+        for (int y = 0; y < B.ty; y++)
+            for (int x = 0; x < B.tx; x++) {
+                if (!Replay::ghost(B, x, y)) continue;
+                const Prov& p = r.prov[b][Replay::index(B, x, y)];
+                HfGhost e;
+                e.dst_block = b;
+                e.dst_cell = (uint32_t)((int64_t)y * B.pitch + x);
+                if (p.blk >= 0) {
+                    e.src_block = p.blk;
+                    e.src_cell = (uint32_t)((int64_t)p.y * c.blocks[p.blk].pitch + p.x) | p.flip;
+                } else {
+                    e.src_block = -(p.cidx + 1);
+                    e.src_cell = p.flip;
+                }
+                fill.push_back(e);
+                HfGhost id;
+                id.dst_block = b;
+                id.dst_cell = e.dst_cell;
+                id.src_block = b;
+                id.src_cell = e.dst_cell;
+                copy.push_back(id);
+            }
+    }
+}
+
+int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+// ---------------------------------------------------------------- launches --
+template <class R>
+HfStepArgs<R> step_args(hf_ctx* c, int item0, int nitems, const void* A, const void* G, void* O)
+{
+    HfStepArgs<R> a;
+    std::memset(&a, 0, sizeof a);
+    a.blocks = c->d_blocks;
+    a.probs = c->d_probs;
+    a.items = c->d_items;
+    a.item0 = item0;
+    a.nitems = nitems;
+    a.A = (const R*)A;
+    a.G = (const R*)G;
+    a.O = (R*)O;
+    a.imax_old = c->imax(c->cur);
+    a.imax_new = c->imax(c->cur ^ 1);
+    a.gmax = c->gmax(c->cur);
+    a.calls = c->d_calls;
+    a.hist = c->d_hist;
+    a.hist_cap = c->hist_cap;
+    a.dt_mode = c->dt_mode;
+    a.gamma = c->gamma;
+    a.cfl = c->cfl;
+    return a;
+}
+
+template <class R>
+int launch_update_t(hf_ctx* c, HfStepArgs<R>& a, bool resid, bool timed)
+{
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (timed) {
+        if (c->ev_free.empty()) {
+            HF_HIP(hipEventCreate(&e0));
+            HF_HIP(hipEventCreate(&e1));
+        } else {
+            e0 = c->ev_free.back().first;
+            e1 = c->ev_free.back().second;
+            c->ev_free.pop_back();
+        }
+        HF_HIP(hipEventRecord(e0, c->stream));
+    }
+    HF_HIP(hf::launch_update<R>(c->scheme, c->limiter, resid, a, c->stream));
+    if (timed) {
+        HF_HIP(hipEventRecord(e1, c->stream));
+        c->ev_pending.emplace_back(e0, e1);
+    }
+    return HF_OK;
+}
+
+int launch_update(hf_ctx* c, int item0, int nitems, const void* A, const void* G, void* O, bool resid,
+                  bool fixed, double dt_step, double dt_pred, bool timed)
+{
+    if (c->prec == HF_FP64) {
+        HfStepArgs<double> a = step_args<double>(c, item0, nitems, A, G, O);
+        a.use_fixed = fixed;
+        a.fixed_step = dt_step;
+        a.fixed_pred = dt_pred;
+        return launch_update_t<double>(c, a, resid, timed);
+    }
+    HfStepArgs<float> a = step_args<float>(c, item0, nitems, A, G, O);
+    a.use_fixed = fixed;
+    a.fixed_step = dt_step;
+    a.fixed_pred = dt_pred;
+    return launch_update_t<float>(c, a, resid, timed);
+}
+
+int launch_ghost(hf_ctx* c, const HfGhost* tab, int n, const void* S, void* D, unsigned long long* gmax,
+                 const GhostBook& book)
+{
+    if (c->prec == HF_FP64)
+        HF_HIP(hf::launch_ghost<double>(tab, n, c->d_blocks, (const double*)S, (double*)D, (const double*)c->d_consts,
+                                        gmax, c->gamma, book, c->stream));
+    else
+        HF_HIP(hf::launch_ghost<float>(tab, n, c->d_blocks, (const float*)S, (float*)D, (const float*)c->d_consts,
+                                       gmax, c->gamma, book, c->stream));
+    return HF_OK;
+}
+
+int launch_scan(hf_ctx* c, int b, const void* I, const void* Gh, int interior_only, unsigned long long* out)
+{
+    if (c->prec == HF_FP64)
+        HF_HIP(hf::launch_scan<double>(c->blocks[b], (const double*)I, (const double*)Gh, interior_only, c->gamma, out,
+                                       c->stream));
+    else
+        HF_HIP(hf::launch_scan<float>(c->blocks[b], (const float*)I, (const float*)Gh, interior_only, c->gamma, out,
+                                      c->stream));
+    return HF_OK;
+}
+
+GhostBook no_book()
+{
+    GhostBook bk;
+    std::memset(&bk, 0, sizeof bk);
+    return bk;
+}
+
+// Copy every ghost cell from S to D (same layout).
+int copy_ghosts(hf_ctx* c, const void* S, void* D)
+{
+    return launch_ghost(c, c->d_copy, c->ncopy, S, D, c->d_scan + 2 * c->nb, no_book());
+}
+
+// Full TimeStep over problem p: interior and ghosts of block q come from
+// src(q); returns the counted dt (timestep.cpp:29-68).
+template <class F>
+int full_dt(hf_ctx* c, int p, F src, double* dt)
+{
+    const HfProbDev& P = c->probs[p];
+    HF_HIP(hipMemsetAsync(c->d_scan, 0, sizeof(unsigned long long) * 2 * c->nb, c->stream));
+    for (int q = P.first; q < P.first + P.nb; q++) {
+        const void* s = src(q);
+        int rc = launch_scan(c, q, s, s, 0, c->d_scan + 2 * q);
+        if (rc) return rc;
+    }
+    std::vector<unsigned long long> mx(2 * c->nb);
+    HF_HIP(hipMemcpyAsync(mx.data(), c->d_scan, sizeof(unsigned long long) * 2 * c->nb, hipMemcpyDeviceToHost,
+                          c->stream));
+    HF_HIP(hipStreamSynchronize(c->stream));
+    double dtx = 1e16, dty = 1e16;
+    for (int q = P.first; q < P.first + P.nb; q++) {
+        dtx = smin(dtx, c->blocks[q].dx / bitsd(mx[2 * q]));
+        dty = smin(dty, c->blocks[q].dy / bitsd(mx[2 * q + 1]));
+    }
+    const int64_t k = ++c->calls[p];
+    const double cfl_eff = (k <= 10) ? 0.2 * c->cfl : c->cfl;
+    *dt = cfl_eff * smin(dtx, dty);
+    if (p == 0 && k >= 1 && k <= c->hist_cap)
+        HF_HIP(hipMemcpy(c->d_hist + (k - 1), dt, sizeof(double), hipMemcpyHostToDevice));
+    return HF_OK;
+}
+
+// Make imax[cur] describe buf[cur]'s interior and zero gmax[cur].
+int ensure_partials(hf_ctx* c)
+{
+    if (c->partials_valid) return HF_OK;
+    HF_HIP(hipMemsetAsync(c->imax(c->cur), 0, sizeof(unsigned long long) * 2 * c->nb, c->stream));
+    HF_HIP(hipMemsetAsync(c->gmax(c->cur), 0, sizeof(unsigned long long) * 2 * c->nb, c->stream));
+    for (int b = 0; b < c->nb; b++) {
+        int rc = launch_scan(c, b, c->buf[c->cur], c->buf[c->cur], 1, c->imax(c->cur) + 2 * b);
+        if (rc) return rc;
+    }
+    c->partials_valid = true;
+    return HF_OK;
+}
+
+// Queue nsteps device-resident steps starting from buffer parity c0.
+int enqueue_steps(hf_ctx* c, int nsteps, bool timed)
+{
+    const int c0 = c->cur;
+    for (int s = 0; s < nsteps; s++) {
+        const int k = c->cur;
+        void* A = c->buf[k];
+        void* B = c->buf[k ^ 1];
+        GhostBook bk;
+        bk.calls = c->d_calls;
+        bk.probs = c->d_probs;
+        bk.nprob = c->np;
+        bk.bump = 1;
+        bk.clear0 = c->gmax(k ^ 1);
+        bk.clear1 = c->imax(k ^ 1);
+        bk.nclear = 2 * c->nb;
+        int rc = launch_ghost(c, c->d_fill, c->nfill, A, B, c->gmax(k), bk);
+        if (rc) return rc;
+        for (const auto& r : c->rounds) {
+            rc = launch_update(c, r.first, r.second, A, B, B, false, false, 0.0, 0.0, timed);
+            if (rc) return rc;
+        }
+        c->cur ^= 1;
+    }
+    c->cur = c0;
+    return HF_OK;
+}
+
+template <class R>
+void to_device_layout(const hf_ctx* c, int b, const double* src, std::vector<R>& dst)
+{
+    const HfBlockDev& B = c->blocks[b];
+    dst.assign((size_t)B.plane * 4, R(0));
+    for (int v = 0; v < 4; v++)
+        for (int y = 0; y < B.ty; y++)
+            for (int x = 0; x < B.tx; x++)
+                dst[(size_t)v * B.plane + (size_t)y * B.pitch + x] = (R)src[((size_t)v * B.ty + y) * B.tx + x];
+}
+
+template <class R>
+void from_device_layout(const hf_ctx* c, int b, const std::vector<R>& src, double* dst)
+{
+    const HfBlockDev& B = c->blocks[b];
+    for (int v = 0; v < 4; v++)
+        for (int y = 0; y < B.ty; y++)
+            for (int x = 0; x < B.tx; x++)
+                dst[((size_t)v * B.ty + y) * B.tx + x] = (double)src[(size_t)v * B.plane + (size_t)y * B.pitch + x];
+}
+
+int download_block(hf_ctx* c, int b, const void* base, int64_t off, double* out)
+{
+    const HfBlockDev& B = c->blocks[b];
+    HF_HIP(hipStreamSynchronize(c->stream));
+    if (c->prec == HF_FP64) {
+        std::vector<double> tmp((size_t)B.plane * 4);
+        HF_HIP(hipMemcpy(tmp.data(), (const double*)base + off, tmp.size() * 8, hipMemcpyDeviceToHost));
+        from_device_layout<double>(c, b, tmp, out);
+    } else {
+        std::vector<float> tmp((size_t)B.plane * 4);
+        HF_HIP(hipMemcpy(tmp.data(), (const float*)base + off, tmp.size() * 4, hipMemcpyDeviceToHost));
+        from_device_layout<float>(c, b, tmp, out);
+    }
+    return HF_OK;
+}
+
+void destroy_graphs(hf_ctx* c)
+{
+    for (auto& kv : c->graphs) {
+        hipGraphExecDestroy(kv.second.second);
+        hipGraphDestroy(kv.second.first);
+    }
+    c->graphs.clear();
+}
+
+}  // namespace
+
+// =================================================================== C ABI ==
+
+extern "C" {
+
+const char* hf_last_error(void) { return g_err.c_str(); }
+
+const char* hf_version(void) { return "hyperflow_amd 0.1 gfx950"; }
+
+int hf_create(const hf_case_desc* desc, hf_ctx** out)
+{
+    if (!out) return fail(HF_EINVAL, "hf_create: null output pointer");
+    *out = nullptr;
+    int rc = validate(desc);
+    if (rc) return rc;
+    std::unique_ptr<hf_ctx> c(new hf_ctx());
+    c->scheme = desc->scheme;
+    c->limiter = desc->limiter;
+    c->prec = desc->precision;
+    c->dt_mode = desc->dt_mode;
+    c->gamma = desc->gamma;
+    c->cfl = desc->cfl;
+    c->g = c->scheme == HF_MUSCL_HANCOCK ? 2 : 1;
+    c->nb = desc->nblocks;
+    c->esize = c->prec == HF_FP64 ? 8 : 4;
+    c->bdesc.assign(desc->blocks, desc->blocks + desc->nblocks);
+    c->np = c->bdesc.back().problem + 1;
+
+    // block layout: planes [var][ty][pitch], pitch a multiple of 32 elements
+    int64_t off = 0;
+    for (int b = 0; b < c->nb; b++) {
+        const hf_block_desc& D = c->bdesc[b];
+        HfBlockDev B;
+        std::memset(&B, 0, sizeof B);
+        B.nx = D.nx;
+        B.ny = D.ny;
+        B.g = c->g;
+        B.tx = D.nx + 2 * c->g;
+        B.ty = D.ny + 2 * c->g;
+        B.pitch = (int32_t)round_up(B.tx, 32);
+        B.plane = (int64_t)B.pitch * B.ty;
+        B.off = off;
+        B.problem = D.problem;
+        B.dx = (D.x_right - D.x_left) / D.nx;     // MeshBlock::calculate_dx (mesh_block.cpp:227-230)
+        B.dy = (D.y_top - D.y_bottom) / D.ny;
+        off = round_up(off + 4 * B.plane, 64);
+        c->blocks.push_back(B);
+        c->resid_elems = std::max<int64_t>(c->resid_elems, 4 * B.plane);
+    }
+    c->total = off;
+    // problems
+    for (int b = 0; b < c->nb; b++) {
+        const int p = c->blocks[b].problem;
+        if ((int)c->probs.size() <= p) c->probs.push_back(HfProbDev{b, 0, 0, 0});
+        c->blocks[b].lb = c->probs[p].nb++;
+    }
+    for (auto& P : c->probs) P.cps = c->scheme == HF_MUSCL_HANCOCK ? 1 + P.nb : 1;
+    c->calls.assign(c->np, 0);
+
+    // work items: rounds of one block per problem (compat MUSCL) or one round
+    const int W = c->g == 2 ? 60 : 62;
+    const int rows = desc->tile_rows > 0 ? desc->tile_rows : 16;
+    int maxnb = 0;
+    for (auto& P : c->probs) maxnb = std::max(maxnb, P.nb);
+    const bool per_block_rounds = c->scheme == HF_MUSCL_HANCOCK && c->dt_mode == HF_DT_COMPAT && maxnb > 1;
+    const int nrounds = per_block_rounds ? maxnb : 1;
+    c->item_begin.assign(c->nb, 0);
+    c->item_count.assign(c->nb, 0);
+    for (int r = 0; r < nrounds; r++) {
+        const int first = (int)c->items.size();
+        for (int p = 0; p < c->np; p++) {
+            const HfProbDev& P = c->probs[p];
+            for (int lb = 0; lb < P.nb; lb++) {
+                if (per_block_rounds && lb != r) continue;
+                const int b = P.first + lb;
+                const HfBlockDev& B = c->blocks[b];
+                c->item_begin[b] = (int)c->items.size();
+                const int strips = (B.nx + W - 1) / W;
+                for (int y = 0; y < B.ny; y += rows)
+                    for (int s = 0; s < strips; s++)
+                        c->items.push_back(HfItem{b, s * W, c->g + y, std::min(rows, B.ny - y)});
+                c->item_count[b] = (int)c->items.size() - c->item_begin[b];
+            }
+        }
+        c->rounds.emplace_back(first, (int)c->items.size() - first);
+    }
+
+    std::vector<HfGhost> fill, copy;
+    build_tables(*c, fill, copy);
+    c->nfill = (int)fill.size();
+    c->ncopy = (int)copy.size();
+
+    hf_ctx* C = c.get();
+    auto cleanup_fail = [&](int code) {
+        hf_destroy(c.release());
+        return code;
+    };
+#define HF_HIPC(call)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess) {                                                               \
+            fail(HF_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));                 \
+            return cleanup_fail(HF_EHIP);                                                     \
+        }                                                                                     \
+    } while (0)
+    HF_HIPC(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
+    for (int i = 0; i < 2; i++) {
+        HF_HIPC(hipMalloc(&C->buf[i], (size_t)C->total * C->esize));
+        HF_HIPC(hipMemset(C->buf[i], 0, (size_t)C->total * C->esize));
+    }
+    HF_HIPC(hipMalloc(&C->d_blocks, sizeof(HfBlockDev) * C->nb));
+    HF_HIPC(hipMemcpy(C->d_blocks, C->blocks.data(), sizeof(HfBlockDev) * C->nb, hipMemcpyHostToDevice));
+    HF_HIPC(hipMalloc(&C->d_probs, sizeof(HfProbDev) * C->np));
+    HF_HIPC(hipMemcpy(C->d_probs, C->probs.data(), sizeof(HfProbDev) * C->np, hipMemcpyHostToDevice));
+    HF_HIPC(hipMalloc(&C->d_items, sizeof(HfItem) * C->items.size()));
+    HF_HIPC(hipMemcpy(C->d_items, C->items.data(), sizeof(HfItem) * C->items.size(), hipMemcpyHostToDevice));
+    HF_HIPC(hipMalloc(&C->d_fill, sizeof(HfGhost) * fill.size()));
+    HF_HIPC(hipMemcpy(C->d_fill, fill.data(), sizeof(HfGhost) * fill.size(), hipMemcpyHostToDevice));
+    HF_HIPC(hipMalloc(&C->d_copy, sizeof(HfGhost) * copy.size()));
+    HF_HIPC(hipMemcpy(C->d_copy, copy.data(), sizeof(HfGhost) * copy.size(), hipMemcpyHostToDevice));
+    {
+        std::vector<double> cd(16 * (size_t)C->nb);
+        for (int b = 0; b < C->nb; b++)
+            for (int s = 0; s < 4; s++)
+                for (int v = 0; v < 4; v++) cd[(b * 4 + s) * 4 + v] = C->bdesc[b].state[s][v];
+        if (C->prec == HF_FP64) {
+            HF_HIPC(hipMalloc(&C->d_consts, cd.size() * 8));
+            HF_HIPC(hipMemcpy(C->d_consts, cd.data(), cd.size() * 8, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> cf(cd.begin(), cd.end());
+            HF_HIPC(hipMalloc(&C->d_consts, cf.size() * 4));
+            HF_HIPC(hipMemcpy(C->d_consts, cf.data(), cf.size() * 4, hipMemcpyHostToDevice));
+        }
+    }
+    HF_HIPC(hipMalloc(&C->d_imax, sizeof(unsigned long long) * 4 * C->nb));
+    HF_HIPC(hipMalloc(&C->d_gmax, sizeof(unsigned long long) * 4 * C->nb));
+    HF_HIPC(hipMalloc(&C->d_scan, sizeof(unsigned long long) * 4 * C->nb));
+    HF_HIPC(hipMemset(C->d_imax, 0, sizeof(unsigned long long) * 4 * C->nb));
+    HF_HIPC(hipMemset(C->d_gmax, 0, sizeof(unsigned long long) * 4 * C->nb));
+    HF_HIPC(hipMalloc(&C->d_calls, sizeof(uint32_t) * C->np));
+    HF_HIPC(hipMemset(C->d_calls, 0, sizeof(uint32_t) * C->np));
+    HF_HIPC(hipHostMalloc((void**)&C->h_calls_pinned, sizeof(uint32_t) * C->np, hipHostMallocDefault));
+    HF_HIPC(hipEventCreateWithFlags(&C->calls_copied, hipEventDisableTiming));
+    HF_HIPC(hipEventRecord(C->calls_copied, C->stream));
+    HF_HIPC(hipMalloc(&C->d_hist, sizeof(double) * C->hist_cap));
+    HF_HIPC(hipMemset(C->d_hist, 0, sizeof(double) * C->hist_cap));
+    HF_HIPC(hipMalloc(&C->d_part, sizeof(double) * 8 * C->part_wg));
+    HF_HIPC(hipMalloc(&C->d_resid, (size_t)C->resid_elems * C->esize));
+#undef HF_HIPC
+    *out = c.release();
+    return HF_OK;
+}
+
+void hf_destroy(hf_ctx* c)
+{
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    destroy_graphs(c);
+    for (auto& e : c->ev_pending) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
+    for (auto& e : c->ev_free) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
+    for (void* p : {c->buf[0], c->buf[1], (void*)c->d_blocks, (void*)c->d_probs, (void*)c->d_items, (void*)c->d_fill,
+                    (void*)c->d_copy, c->d_consts, (void*)c->d_imax, (void*)c->d_gmax, (void*)c->d_scan,
+                    (void*)c->d_calls, (void*)c->d_hist, (void*)c->d_part, c->d_resid})
+        if (p) hipFree(p);
+    if (c->h_calls_pinned) hipHostFree(c->h_calls_pinned);
+    if (c->calls_copied) hipEventDestroy(c->calls_copied);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int hf_ghost_cells(const hf_ctx* c)
+{
+    if (!c) return fail(HF_EINVAL, "null context");
+    return c->g;
+}
+
+int hf_upload(hf_ctx* c, int b, const double* fields, size_t n)
+{
+    if (!c || !fields) return fail(HF_EINVAL, "hf_upload: null argument");
+    if (b < 0 || b >= c->nb) return fail(HF_EINVAL, "hf_upload: block out of range");
+    const HfBlockDev& B = c->blocks[b];
+    if (n != (size_t)4 * B.tx * B.ty) return fail(HF_EINVAL, "hf_upload: size must be 4*(nx+2g)*(ny+2g)");
+    HF_HIP(hipStreamSynchronize(c->stream));
+    if (c->prec == HF_FP64) {
+        std::vector<double> tmp;
+        to_device_layout<double>(c, b, fields, tmp);
+        HF_HIP(hipMemcpy((double*)c->buf[c->cur] + B.off, tmp.data(), tmp.size() * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> tmp;
+        to_device_layout<float>(c, b, fields, tmp);
+        HF_HIP(hipMemcpy((float*)c->buf[c->cur] + B.off, tmp.data(), tmp.size() * 4, hipMemcpyHostToDevice));
+    }
+    c->ghosts_pending = false;
+    c->partials_valid = false;
+    return HF_OK;
+}
+
+int hf_download(hf_ctx* c, int b, double* fields, size_t n)
+{
+    if (!c || !fields) return fail(HF_EINVAL, "hf_download: null argument");
+    if (b < 0 || b >= c->nb) return fail(HF_EINVAL, "hf_download: block out of range");
+    const HfBlockDev& B = c->blocks[b];
+    if (n != (size_t)4 * B.tx * B.ty) return fail(HF_EINVAL, "hf_download: size must be 4*(nx+2g)*(ny+2g)");
+    return download_block(c, b, c->buf[c->cur], B.off, fields);
+}
+
+int hf_apply_bcs(hf_ctx* c)
+{
+    if (!c) return fail(HF_EINVAL, "null context");
+    void* A = c->buf[c->cur];
+    void* B = c->buf[c->cur ^ 1];
+    HF_HIP(hipMemsetAsync(c->d_scan, 0, sizeof(unsigned long long) * 4 * c->nb, c->stream));
+    int rc = launch_ghost(c, c->d_fill, c->nfill, A, B, c->d_scan, no_book());
+    if (rc) return rc;
+    rc = copy_ghosts(c, B, A);
+    if (rc) return rc;
+    c->ghosts_pending = true;
+    c->partials_valid = false;
+    return HF_OK;
+}
+
+int hf_time_step(hf_ctx* c, int p, double* dt)
+{
+    if (!c || !dt) return fail(HF_EINVAL, "hf_time_step: null argument");
+    if (p < 0 || p >= c->np) return fail(HF_EINVAL, "hf_time_step: problem out of range");
+    void* A = c->buf[c->cur];
+    return full_dt(c, p, [&](int) { return A; }, dt);
+}
+
+int hf_scheme_residual(hf_ctx* c, int b, double* L, size_t n)
+{
+    if (!c || !L) return fail(HF_EINVAL, "hf_scheme_residual: null argument");
+    if (b < 0 || b >= c->nb) return fail(HF_EINVAL, "hf_scheme_residual: block out of range");
+    const HfBlockDev& B = c->blocks[b];
+    if (n != (size_t)4 * B.tx * B.ty) return fail(HF_EINVAL, "hf_scheme_residual: size must be 4*(nx+2g)*(ny+2g)");
+    void* A = c->buf[c->cur];
+    double dt_pred = 0.0;
+    if (c->scheme == HF_MUSCL_HANCOCK) {
+        int rc = full_dt(c, B.problem, [&](int) { return A; }, &dt_pred);   // muscl_hancock.cpp:246
+        if (rc) return rc;
+    }
+    HF_HIP(hipMemsetAsync(c->d_resid, 0, (size_t)c->resid_elems * c->esize, c->stream));
+    int rc = launch_update(c, c->item_begin[b], c->item_count[b], A, A, c->d_resid, true, true, 0.0, dt_pred, false);
+    if (rc) return rc;
+    return download_block(c, b, c->d_resid, 0, L);
+}
+
+int hf_advance(hf_ctx* c, const double* dt)
+{
+    if (!c || !dt) return fail(HF_EINVAL, "hf_advance: null argument");
+    void* A = c->buf[c->cur];
+    void* B = c->buf[c->cur ^ 1];
+    if (!c->ghosts_pending) {
+        int rc = copy_ghosts(c, A, B);
+        if (rc) return rc;
+    }
+    for (int p = 0; p < c->np; p++) {
+        const HfProbDev& P = c->probs[p];
+        for (int q = P.first; q < P.first + P.nb; q++) {
+            double dt_pred = dt[p];
+            if (c->scheme == HF_MUSCL_HANCOCK) {
+                if (c->dt_mode == HF_DT_COMPAT) {
+                    // the predictor's TimeStep sees blocks before q already updated
+                    int rc = full_dt(c, p, [&](int r) { return r < q ? B : A; }, &dt_pred);
+                    if (rc) return rc;
+                } else {
+                    const int64_t k = ++c->calls[p];
+                    if (p == 0 && k <= c->hist_cap)
+                        HF_HIP(hipMemcpy(c->d_hist + (k - 1), &dt_pred, sizeof(double), hipMemcpyHostToDevice));
+                }
+            }
+            int rc = launch_update(c, c->item_begin[q], c->item_count[q], A, B, B, false, true, dt[p], dt_pred,
+                                   c->profile);
+            if (rc) return rc;
+        }
+    }
+    c->cur ^= 1;
+    c->ghosts_pending = false;
+    c->partials_valid = false;
+    return HF_OK;
+}
+
+int hf_run(hf_ctx* c, int nsteps, int flags)
+{
+    if (!c) return fail(HF_EINVAL, "null context");
+    if (nsteps < 0) return fail(HF_EINVAL, "hf_run: nsteps must be >= 0");
+    if (nsteps == 0) return HF_OK;
+    int rc = ensure_partials(c);
+    if (rc) return rc;
+    HF_HIP(hipEventSynchronize(c->calls_copied));   // earlier upload must have consumed the staging buffer
+    for (int p = 0; p < c->np; p++) c->h_calls_pinned[p] = (uint32_t)c->calls[p];
+    HF_HIP(hipMemcpyAsync(c->d_calls, c->h_calls_pinned, sizeof(uint32_t) * c->np, hipMemcpyHostToDevice, c->stream));
+    HF_HIP(hipEventRecord(c->calls_copied, c->stream));
+    if (flags & HF_RUN_GRAPH) {
+        const auto key = std::make_pair(nsteps, c->cur);
+        auto it = c->graphs.find(key);
+        if (it == c->graphs.end()) {
+            hipGraph_t g = nullptr;
+            hipGraphExec_t ge = nullptr;
+            HF_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            rc = enqueue_steps(c, nsteps, false);
+            hipError_t ce = hipStreamEndCapture(c->stream, &g);
+            if (rc) {
+                if (g) hipGraphDestroy(g);
+                return rc;
+            }
+            HF_HIP(ce);
+            HF_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+            it = c->graphs.emplace(key, std::make_pair(g, ge)).first;
+        }
+        HF_HIP(hipGraphLaunch(it->second.second, c->stream));
+    } else {
+        rc = enqueue_steps(c, nsteps, c->profile);
+        if (rc) return rc;
+    }
+    c->cur ^= (nsteps & 1);
+    for (int p = 0; p < c->np; p++) c->calls[p] += (int64_t)nsteps * c->probs[p].cps;
+    c->ghosts_pending = false;
+    return HF_OK;
+}
+
+int hf_synchronize(hf_ctx* c)
+{
+    if (!c) return fail(HF_EINVAL, "null context");
+    HF_HIP(hipStreamSynchronize(c->stream));
+    return HF_OK;
+}
+
+int hf_dt_history(hf_ctx* c, double* out, int max, int* count)
+{
+    if (!c) return fail(HF_EINVAL, "null context");
+    const int64_t n = c->calls[0];
+    if (count) *count = (int)std::min<int64_t>(n, INT32_MAX);
+    if (out && max > 0) {
+        HF_HIP(hipStreamSynchronize(c->stream));
+        const int64_t k = std::min<int64_t>(std::min<int64_t>(n, max), c->hist_cap);
+        if (k > 0) HF_HIP(hipMemcpy(out, c->d_hist, sizeof(double) * k, hipMemcpyDeviceToHost));
+    }
+    return HF_OK;
+}
+
+int hf_calls(hf_ctx* c, int p, int64_t* calls)
+{
+    if (!c || !calls) return fail(HF_EINVAL, "hf_calls: null argument");
+    if (p < 0 || p >= c->np) return fail(HF_EINVAL, "hf_calls: problem out of range");
+    *calls = c->calls[p];
+    return HF_OK;
+}
+
+int hf_profile(hf_ctx* c, int enable)
+{
+    if (!c) return fail(HF_EINVAL, "null context");
+    c->profile = enable != 0;
+    return HF_OK;
+}
+
+int hf_kernel_stats(hf_ctx* c, int64_t* launches, double* total_ms)
+{
+    if (!c) return fail(HF_EINVAL, "null context");
+    HF_HIP(hipStreamSynchronize(c->stream));
+    for (auto& e : c->ev_pending) {
+        float ms = 0.f;
+        HF_HIP(hipEventElapsedTime(&ms, e.first, e.second));
+        c->prof_ms += ms;
+        c->prof_launches++;
+        c->ev_free.push_back(e);
+    }
+    c->ev_pending.clear();
+    if (launches) *launches = c->prof_launches;
+    if (total_ms) *total_ms = c->prof_ms;
+    c->prof_launches = 0;
+    c->prof_ms = 0.0;
+    return HF_OK;
+}
+
+int hf_conservation(hf_ctx* c, int p, double sums[4])
+{
+    if (!c || !sums) return fail(HF_EINVAL, "hf_conservation: null argument");
+    if (p < 0 || p >= c->np) return fail(HF_EINVAL, "hf_conservation: problem out of range");
+    double S[4] = {0, 0, 0, 0}, Cm[4] = {0, 0, 0, 0};
+    std::vector<double> part(8 * (size_t)c->part_wg);
+    const HfProbDev& P = c->probs[p];
+    for (int q = P.first; q < P.first + P.nb; q++) {
+        if (c->prec == HF_FP64)
+            HF_HIP(hf::launch_conserve<double>(c->blocks[q], (const double*)c->buf[c->cur], c->d_part, c->part_wg,
+                                               c->stream));
+        else
+            HF_HIP(hf::launch_conserve<float>(c->blocks[q], (const float*)c->buf[c->cur], c->d_part, c->part_wg,
+                                              c->stream));
+        HF_HIP(hipMemcpyAsync(part.data(), c->d_part, part.size() * 8, hipMemcpyDeviceToHost, c->stream));
+        HF_HIP(hipStreamSynchronize(c->stream));
+        for (int w = 0; w < c->part_wg; w++)
+            for (int k = 0; k < 4; k++) {
+                for (int h = 0; h < 2; h++) {
+                    const double v = part[(w * 4 + k) * 2 + h];
+                    const double t = S[k] + v;
+                    Cm[k] += (std::fabs(S[k]) >= std::fabs(v)) ? ((S[k] - t) + v) : ((v - t) + S[k]);
+                    S[k] = t;
+                }
+            }
+    }
+    for (int k = 0; k < 4; k++) sums[k] = S[k] + Cm[k];
+    return HF_OK;
+}
+
+}  // extern "C"

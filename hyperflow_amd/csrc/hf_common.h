@@ -1,0 +1,75 @@
+// hf_common.h -- structures shared by the host library and the gfx950 kernels.
+#pragma once
+#include <stdint.h>
+
+// One padded block in the state buffers: four planes [var][ty][pitch], x fastest.
+struct HfBlockDev {
+    int64_t off;      // element offset of plane 0 inside the state buffer
+    int64_t plane;    // elements per plane (pitch * ty)
+    int32_t nx, ny, tx, ty;
+    int32_t pitch;    // row pitch in elements
+    int32_t problem;  // owning problem (dt group)
+    int32_t lb;       // block index inside its problem
+    int32_t g;        // ghost layers
+    double dx, dy;
+};
+
+// A wave's share of an update launch: one column strip of one block, `rows`
+// output rows starting at padded row y0.  Lane l owns padded column x0 + l.
+struct HfItem {
+    int32_t block, x0, y0, rows;
+};
+
+// An independent problem: blocks [first, first+nb) share a dt and a TimeStep
+// call counter; cps = TimeStep calls per step (1 Godunov, 1+nb MUSCL).
+struct HfProbDev {
+    int32_t first, nb, cps, pad;
+};
+
+// One ghost cell of the table-driven ghost fill.  The destination gets the
+// source cell's value (read from the pre-fill state) or a Dirichlet constant,
+// with the x/y momentum optionally negated (composed Reflective sides).
+struct HfGhost {
+    int32_t dst_block;
+    uint32_t dst_cell;   // y * pitch + x in the destination block
+    int32_t src_block;   // >= 0: source block; < 0: constant index -src_block-1
+    uint32_t src_cell;   // low 30 bits: y * pitch + x; bit 30: negate rho*u; bit 31: negate rho*v
+};
+
+#define HF_FLIP_U 0x40000000u
+#define HF_FLIP_V 0x80000000u
+#define HF_CELL_MASK 0x3fffffffu
+
+// Arguments of one update launch (Godunov or MUSCL-Hancock sweep kernel).
+template <class R>
+struct HfStepArgs {
+    const HfBlockDev* blocks;
+    const HfProbDev* probs;
+    const HfItem* items;
+    int32_t item0, nitems;
+    const R* A;   // interior source (U^n)
+    const R* G;   // ghost source (filled ghosts)
+    R* O;         // destination: U^{n+1} interior, or the residual L
+    const unsigned long long* imax_old;  // [nblocks][2] max wave speed of the U^n interior
+    unsigned long long* imax_new;        // [nblocks][2] max wave speed of the new interior
+    const unsigned long long* gmax;      // [nblocks][2] max wave speed of the ghosts
+    const uint32_t* calls;               // TimeStep calls per problem (already bumped for this step)
+    double* hist;                        // TimeStep results of problem 0 by call index
+    int32_t hist_cap;
+    int32_t dt_mode;                     // 0 compat, 1 uniform
+    double gamma, cfl;
+    int32_t use_fixed;                   // 1: dt_step/dt_pred below instead of the device partials
+    int32_t pad;
+    double fixed_step, fixed_pred;
+};
+
+// Step bookkeeping done by the first workgroup of a ghost-fill launch.
+struct GhostBook {
+    uint32_t* calls;
+    const HfProbDev* probs;
+    int32_t nprob;
+    int32_t bump;                 // add cps to every problem's counter
+    unsigned long long* clear0;   // arrays to zero (next step's partials)
+    unsigned long long* clear1;
+    int32_t nclear;               // elements in each
+};

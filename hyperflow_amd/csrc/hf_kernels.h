@@ -1,0 +1,406 @@
+// hf_kernels.h -- gfx950 kernels of the block-Cartesian Euler step.
+//
+// Update kernels ("sweeps"): each 64-lane wave owns a strip of padded columns
+// (lane l <-> column x0+l) and walks down `rows` output rows of one block.
+// The reconstruction window, face states, both HLLC fluxes and the update of
+// a row live in registers; x neighbours come from cross-lane shuffles, y
+// neighbours from the previous iteration.  Every conserved value is read once
+// from HBM (plus the 2-lane halo at each strip edge and 2 halo rows per
+// strip) and every output written once; nothing intermediate goes to memory.
+// The new state's max wave speeds (for the next dt) are reduced in the same
+// pass: wave shuffles, then one 64-bit atomic max per wave.
+//
+// The interior is read from A and the filled ghosts from G (the ghost fill
+// writes them into the destination buffer), so no launch ever reads a cell
+// that another wave of the same step writes.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "hf_common.h"
+#include "hf_physics.h"
+
+namespace hf {
+
+template <class R>
+__device__ __forceinline__ void load_cell(const R* __restrict__ base, int64_t plane, R u[4])
+{
+    u[0] = base[0];
+    u[1] = base[plane];
+    u[2] = base[2 * plane];
+    u[3] = base[3 * plane];
+}
+
+template <class R>
+__device__ __forceinline__ R shfl_up1(R x) { return __shfl_up(x, 1, 64); }
+template <class R>
+__device__ __forceinline__ R shfl_dn1(R x) { return __shfl_down(x, 1, 64); }
+
+// Load row y of this lane's column: ghost rows/columns come from the ghost
+// source, interior cells from the interior source.
+template <class R>
+__device__ __forceinline__ void load_row(const R* __restrict__ Acol, const R* __restrict__ Gcol, bool gcol, int y,
+                                         int g, int ty, int pitch, int64_t plane, R u[4])
+{
+    const bool gh = gcol || y < g || y >= ty - g;
+    load_cell<R>((gh ? Gcol : Acol) + (int64_t)y * pitch, plane, u);
+}
+
+// ---------------------------------------------------------- MUSCL-Hancock --
+// MUSCLHancockScheme::operator() (scheme/muscl_hancock/muscl_hancock.cpp:41-55)
+// fused with ForwardEulerODESolver (scheme/ode/forward_euler/forward_euler.cpp:25-30).
+// Output columns per wave: 60 (lanes 2..61); lanes 0,1,62,63 are halo.
+template <class M, int LIM, bool RESID>
+__global__ __launch_bounds__(256) void k_muscl(const HfStepArgs<typename M::R> a)
+{
+    typedef typename M::R R;
+    const int lane = threadIdx.x & 63;
+    const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (w >= a.nitems) return;
+    const HfItem it = a.items[a.item0 + w];
+
+    const HfBlockDev bd = a.blocks[it.block];
+    const int xraw = it.x0 + lane;
+    const bool valid = xraw < bd.tx;
+    const int xc = valid ? xraw : bd.tx - 1;
+    const bool gcol = xc < 2 || xc >= bd.tx - 2;
+    const R* Acol = a.A + bd.off + xc;
+    const R* Gcol = a.G + bd.off + xc;
+    const int pitch = bd.pitch, ty = bd.ty;
+    const int64_t plane = bd.plane;
+#define ROW(y, u) load_row<R>(Acol, Gcol, gcol, (y), 2, ty, pitch, plane, (u))
+    const bool out = valid && lane >= 2 && lane < 62 && xraw >= 2 && xraw < bd.tx - 2;
+
+    double dt_step, dt_pred;
+    long long kstep, kpred;
+    if (a.use_fixed) {
+        dt_step = a.fixed_step;
+        dt_pred = a.fixed_pred;
+        kstep = kpred = 0;
+    } else {
+        step_dts<R>(a, it.block, bd, 1, dt_step, dt_pred, kstep, kpred);
+        if (bd.problem == 0 && it.x0 == 0 && it.y0 == bd.g && lane == 0) {
+            if (bd.lb == 0 && kstep >= 1 && kstep <= a.hist_cap) a.hist[kstep - 1] = dt_step;
+            if (kpred >= 1 && kpred <= a.hist_cap) a.hist[kpred - 1] = dt_pred;
+        }
+    }
+    const R hx = R((0.5 * dt_pred) / bd.dx);     // half_dt_dx, muscl_hancock.cpp:248
+    const R hy = R((0.5 * dt_pred) / bd.dy);
+    const R rdx = R(1.0 / bd.dx);                  // one_dx, muscl_hancock.cpp:352
+    const R rdy = R(1.0 / bd.dy);
+    const R dtR = R(dt_step);
+    const R gam = R(a.gamma), g1 = R(a.gamma - 1.0);
+
+    const int y0 = it.y0;
+    R um[4], uc[4], up[4], un[4];
+    ROW(y0 - 2, um);
+    ROW(y0 - 1, uc);
+    ROW(y0, up);
+    ROW(y0 + 1, un);
+
+    // warm-up: T' of cell y0-1, (B', T') of cell y0, flux through face y0-1/2
+    R Gs[4], Tc[4];
+    {
+        R lo[4], hi[4];
+        reconstruct<M, LIM, false>(um, uc, up, hy, g1, lo, hi);   // cell y0-1
+        R lo2[4];
+        reconstruct<M, LIM, false>(uc, up, un, hy, g1, lo2, Tc);  // cell y0
+        const State<R> sl = make_state<M>(hi, gam, g1);
+        const State<R> sr = make_state<M>(lo2, gam, g1);
+        hllc<M, false>(sl, sr, Gs);
+    }
+    // window: uc = U(y), up = U(y+1)
+    for (int k = 0; k < 4; k++) { uc[k] = up[k]; up[k] = un[k]; }
+
+    R* Ob = RESID ? a.O : a.O + bd.off;
+    double mx = 0.0, my = 0.0;
+    for (int r = 0; r < it.rows; r++) {
+        const int y = y0 + r;
+        ROW(y + 2, un);
+
+        // y direction: reconstruct cell y+1, flux through face y+1/2
+        R Bn[4], Tn[4], Gn[4];
+        reconstruct<M, LIM, false>(uc, up, un, hy, g1, Bn, Tn);
+        {
+            const State<R> sl = make_state<M>(Tc, gam, g1);
+            const State<R> sr = make_state<M>(Bn, gam, g1);
+            hllc<M, false>(sl, sr, Gn);
+        }
+
+        // x direction: reconstruct cell (x, y) from lane neighbours
+        R ul[4], ur[4], Lf[4], Rf[4], Rl[4], Fw[4], Fe[4];
+        for (int k = 0; k < 4; k++) { ul[k] = shfl_up1(uc[k]); ur[k] = shfl_dn1(uc[k]); }
+        reconstruct<M, LIM, true>(ul, uc, ur, hx, g1, Lf, Rf);
+        for (int k = 0; k < 4; k++) Rl[k] = shfl_up1(Rf[k]);
+        {
+            const State<R> sl = make_state<M>(Rl, gam, g1);
+            const State<R> sr = make_state<M>(Lf, gam, g1);
+            hllc<M, true>(sl, sr, Fw);          // face x-1/2
+        }
+        for (int k = 0; k < 4; k++) Fe[k] = shfl_dn1(Fw[k]);   // face x+1/2
+
+        // calculate_spatial_update (muscl_hancock.cpp:364-371) and (dt*L)+U
+        R Ln[4], Un[4];
+        for (int k = 0; k < 4; k++) {
+            Ln[k] = rdx * (Fw[k] - Fe[k]) + rdy * (Gs[k] - Gn[k]);
+            Un[k] = dtR * Ln[k] + uc[k];
+        }
+        if (out) {
+            R* o = Ob + (int64_t)y * pitch + xraw;
+            for (int k = 0; k < 4; k++) o[k * plane] = RESID ? Ln[k] : Un[k];
+            if (!RESID) {
+                double dx_, dy_;
+                wave_speeds<M>(Un[0], Un[1], Un[2], Un[3], gam, g1, dx_, dy_);
+                mx = dmax_nan(mx, dx_);
+                my = dmax_nan(my, dy_);
+            }
+        }
+        for (int k = 0; k < 4; k++) { Gs[k] = Gn[k]; Tc[k] = Tn[k]; uc[k] = up[k]; up[k] = un[k]; }
+    }
+    if (!RESID) {
+        mx = wave_max(mx);
+        my = wave_max(my);
+        if (lane == 0) {
+            atomicMax(&a.imax_new[2 * it.block], dbits(mx));
+            atomicMax(&a.imax_new[2 * it.block + 1], dbits(my));
+        }
+    }
+#undef ROW
+}
+
+// ---------------------------------------------------------------- Godunov --
+// GodunovScheme::operator() (scheme/godunov/godunov.cpp:27-118) fused with
+// the forward Euler update.  Each cell's primitives and sound speed are
+// computed once and shared by its four faces.  Output columns per wave: 62.
+template <class M, bool RESID>
+__global__ __launch_bounds__(256) void k_godunov(const HfStepArgs<typename M::R> a)
+{
+    typedef typename M::R R;
+    const int lane = threadIdx.x & 63;
+    const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (w >= a.nitems) return;
+    const HfItem it = a.items[a.item0 + w];
+
+    const HfBlockDev bd = a.blocks[it.block];
+    const int xraw = it.x0 + lane;
+    const bool valid = xraw < bd.tx;
+    const int xc = valid ? xraw : bd.tx - 1;
+    const bool gcol = xc < 1 || xc >= bd.tx - 1;
+    const R* Acol = a.A + bd.off + xc;
+    const R* Gcol = a.G + bd.off + xc;
+    const int pitch = bd.pitch, ty = bd.ty;
+    const int64_t plane = bd.plane;
+#define ROW(y, u) load_row<R>(Acol, Gcol, gcol, (y), 1, ty, pitch, plane, (u))
+    const bool out = valid && lane >= 1 && lane < 63 && xraw >= 1 && xraw < bd.tx - 1;
+
+    double dt_step, dt_pred;
+    long long kstep, kpred;
+    if (a.use_fixed) {
+        dt_step = a.fixed_step;
+        kstep = 0;
+    } else {
+        step_dts<R>(a, it.block, bd, 0, dt_step, dt_pred, kstep, kpred);
+        if (bd.problem == 0 && bd.lb == 0 && it.x0 == 0 && it.y0 == bd.g && lane == 0 &&
+            kstep >= 1 && kstep <= a.hist_cap)
+            a.hist[kstep - 1] = dt_step;
+    }
+    const R rdx = R(1.0 / bd.dx);                  // godunov.cpp:97-98
+    const R rdy = R(1.0 / bd.dy);
+    const R dtR = R(dt_step);
+    const R gam = R(a.gamma), g1 = R(a.gamma - 1.0);
+
+    const int y0 = it.y0;
+    R ub[4], uc[4];
+    ROW(y0 - 1, ub);
+    ROW(y0, uc);
+    State<R> sb = make_state<M>(ub, gam, g1);
+    State<R> sc = make_state<M>(uc, gam, g1);
+    R Gs[4];
+    hllc<M, false>(sb, sc, Gs);                    // face y0-1/2
+
+    R* Ob = RESID ? a.O : a.O + bd.off;
+    double mx = 0.0, my = 0.0;
+    for (int r = 0; r < it.rows; r++) {
+        const int y = y0 + r;
+        R un[4];
+        ROW(y + 1, un);
+        const State<R> sn = make_state<M>(un, gam, g1);
+        R Gn[4], Fw[4], Fe[4];
+        hllc<M, false>(sc, sn, Gn);                // face y+1/2
+
+        State<R> sw;
+        sw.U0 = shfl_up1(sc.U0); sw.U1 = shfl_up1(sc.U1); sw.U2 = shfl_up1(sc.U2); sw.U3 = shfl_up1(sc.U3);
+        sw.u = shfl_up1(sc.u); sw.v = shfl_up1(sc.v); sw.p = shfl_up1(sc.p); sw.E = shfl_up1(sc.E);
+        sw.a = shfl_up1(sc.a);
+        hllc<M, true>(sw, sc, Fw);                 // face x-1/2
+        for (int k = 0; k < 4; k++) Fe[k] = shfl_dn1(Fw[k]);
+
+        const R U[4] = {sc.U0, sc.U1, sc.U2, sc.U3};
+        R Ln[4], Un[4];
+        for (int k = 0; k < 4; k++) {
+            Ln[k] = rdx * (Fw[k] - Fe[k]) + rdy * (Gs[k] - Gn[k]);
+            Un[k] = dtR * Ln[k] + U[k];
+        }
+        if (out) {
+            R* o = Ob + (int64_t)y * pitch + xraw;
+            for (int k = 0; k < 4; k++) o[k * plane] = RESID ? Ln[k] : Un[k];
+            if (!RESID) {
+                double dx_, dy_;
+                wave_speeds<M>(Un[0], Un[1], Un[2], Un[3], gam, g1, dx_, dy_);
+                mx = dmax_nan(mx, dx_);
+                my = dmax_nan(my, dy_);
+            }
+        }
+        for (int k = 0; k < 4; k++) Gs[k] = Gn[k];
+        sc = sn;
+    }
+    if (!RESID) {
+        mx = wave_max(mx);
+        my = wave_max(my);
+        if (lane == 0) {
+            atomicMax(&a.imax_new[2 * it.block], dbits(mx));
+            atomicMax(&a.imax_new[2 * it.block + 1], dbits(my));
+        }
+    }
+#undef ROW
+}
+
+// ------------------------------------------------------------- ghost fill --
+// Mesh::apply_bcs (mesh/mesh/mesh.cpp:31-39) as a gather: the host replays the
+// reference's sequential side order symbolically and records, for every
+// ghost cell, which pre-fill cell (or Dirichlet constant) it ends up holding
+// and which momentum components were negated on the way.  Reads come from S,
+// writes go to D, so the fill is race-free in one launch.  Also reduces the
+// ghosts' max wave speeds per block, and (step mode) advances each problem's
+// TimeStep call counter and clears the partial maxima of the next step.
+
+template <class M>
+__global__ __launch_bounds__(256) void k_ghost(const HfGhost* __restrict__ tab, int n,
+                                               const HfBlockDev* __restrict__ blocks,
+                                               const typename M::R* __restrict__ S, typename M::R* __restrict__ D,
+                                               const typename M::R* __restrict__ consts, unsigned long long* gmax,
+                                               double gamma, GhostBook book)
+{
+    typedef typename M::R R;
+    if (blockIdx.x == 0) {
+        for (int p = threadIdx.x; p < book.nprob && book.bump; p += blockDim.x)
+            book.calls[p] += (uint32_t)book.probs[p].cps;
+        for (int i = threadIdx.x; i < book.nclear; i += blockDim.x) {
+            if (book.clear0) book.clear0[i] = 0ull;
+            if (book.clear1) book.clear1[i] = 0ull;
+        }
+    }
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    double dx_ = 0.0, dy_ = 0.0;
+    int blk = -1;
+    if (gid < n) {
+        const HfGhost e = tab[gid];
+        R u[4];
+        if (e.src_block >= 0) {
+            const HfBlockDev sb = blocks[e.src_block];
+            const R* p = S + sb.off + (e.src_cell & HF_CELL_MASK);
+            load_cell<R>(p, sb.plane, u);
+        } else {
+            const R* c = consts + 4 * (-e.src_block - 1);
+            u[0] = c[0]; u[1] = c[1]; u[2] = c[2]; u[3] = c[3];
+        }
+        if (e.src_cell & HF_FLIP_U) u[1] = -u[1];
+        if (e.src_cell & HF_FLIP_V) u[2] = -u[2];
+        const HfBlockDev db = blocks[e.dst_block];
+        R* q = D + db.off + e.dst_cell;
+        q[0] = u[0];
+        q[db.plane] = u[1];
+        q[2 * db.plane] = u[2];
+        q[3 * db.plane] = u[3];
+        double x, y;
+        wave_speeds<M>(u[0], u[1], u[2], u[3], R(gamma), R(gamma - 1.0), x, y);
+        dx_ = dmax_nan(0.0, x);
+        dy_ = dmax_nan(0.0, y);
+        blk = e.dst_block;
+    }
+    // one atomic per wave when the wave covers a single block (the table is
+    // grouped by destination block), per lane otherwise
+    const int b0 = __shfl(blk, 0, 64);
+    const bool uniform = __all(blk == b0 || blk < 0);
+    if (uniform) {
+        const double wx = wave_max(dx_), wy = wave_max(dy_);
+        if (lane == 0 && b0 >= 0) {
+            atomicMax(&gmax[2 * b0], dbits(wx));
+            atomicMax(&gmax[2 * b0 + 1], dbits(wy));
+        }
+    } else if (blk >= 0) {
+        atomicMax(&gmax[2 * blk], dbits(dx_));
+        atomicMax(&gmax[2 * blk + 1], dbits(dy_));
+    }
+}
+
+// -------------------------------------------------------------- wave scan --
+// Max of |u|+c and |v|+c over cells of one block (TimeStep::operator(),
+// timestep.cpp:37-55).  Interior cells are read from I, ghost cells from Gh;
+// with interior_only the ghosts are skipped.
+template <class M>
+__global__ __launch_bounds__(256) void k_scan(HfBlockDev bd, const typename M::R* __restrict__ I,
+                                              const typename M::R* __restrict__ Gh, int interior_only,
+                                              double gamma, unsigned long long* outmax)
+{
+    typedef typename M::R R;
+    const int64_t n = (int64_t)bd.tx * bd.ty;
+    double mx = 0.0, my = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int y = (int)(i / bd.tx), x = (int)(i % bd.tx);
+        const bool gh = x < bd.g || x >= bd.tx - bd.g || y < bd.g || y >= bd.ty - bd.g;
+        if (gh && interior_only) continue;
+        R u[4];
+        load_cell<R>((gh ? Gh : I) + bd.off + (int64_t)y * bd.pitch + x, bd.plane, u);
+        double dx_, dy_;
+        wave_speeds<M>(u[0], u[1], u[2], u[3], R(gamma), R(gamma - 1.0), dx_, dy_);
+        mx = dmax_nan(mx, dx_);
+        my = dmax_nan(my, dy_);
+    }
+    mx = wave_max(mx);
+    my = wave_max(my);
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&outmax[0], dbits(mx));
+        atomicMax(&outmax[1], dbits(my));
+    }
+}
+
+// --------------------------------------------------- conservation sums --
+// Neumaier-compensated FP64 partial sums of the four conserved variables
+// over a block's interior; one (sum, compensation) pair per workgroup and
+// variable, combined on the host in workgroup order.
+template <class R>
+__global__ __launch_bounds__(256) void k_conserve(HfBlockDev bd, const R* __restrict__ A, double* part)
+{
+    __shared__ double ss[4][256];
+    __shared__ double cc[4][256];
+    double s[4] = {0, 0, 0, 0}, c[4] = {0, 0, 0, 0};
+    const int64_t n = (int64_t)bd.nx * bd.ny;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int y = (int)(i / bd.nx) + bd.g, x = (int)(i % bd.nx) + bd.g;
+        const R* p = A + bd.off + (int64_t)y * bd.pitch + x;
+        for (int k = 0; k < 4; k++) {
+            const double v = (double)p[k * bd.plane];
+            const double t = s[k] + v;
+            c[k] += (__builtin_fabs(s[k]) >= __builtin_fabs(v)) ? ((s[k] - t) + v) : ((v - t) + s[k]);
+            s[k] = t;
+        }
+    }
+    for (int k = 0; k < 4; k++) { ss[k][threadIdx.x] = s[k]; cc[k][threadIdx.x] = c[k]; }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const int k = threadIdx.x;
+        double S = 0.0, Cm = 0.0;
+        for (int i = 0; i < (int)blockDim.x; i++) {
+            const double v = ss[k][i];
+            const double t = S + v;
+            Cm += (__builtin_fabs(S) >= __builtin_fabs(v)) ? ((S - t) + v) : ((v - t) + S);
+            S = t;
+            Cm += cc[k][i];
+        }
+        part[(blockIdx.x * 4 + k) * 2] = S;
+        part[(blockIdx.x * 4 + k) * 2 + 1] = Cm;
+    }
+}
+
+}  // namespace hf

@@ -1,0 +1,58 @@
+// hf_kernels_inst.h -- launcher definitions, included once per arithmetic
+// type by hf_kernels_f64.hip / hf_kernels_f32.hip with HF_MATH set.
+#pragma once
+#include "hf_kernels.h"
+#include "hf_launch.h"
+
+namespace hf {
+
+typedef HF_MATH Math;
+typedef Math::R Real;
+
+template <>
+hipError_t launch_update<Real>(int scheme, int limiter, bool resid, const HfStepArgs<Real>& a, hipStream_t s)
+{
+    if (a.nitems <= 0) return hipSuccess;
+    const dim3 block(256), grid((a.nitems + 3) / 4);
+    if (scheme == 0) {
+        if (resid) hipLaunchKernelGGL((k_godunov<Math, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((k_godunov<Math, false>), grid, block, 0, s, a);
+    } else if (limiter == 0) {
+        if (resid) hipLaunchKernelGGL((k_muscl<Math, 0, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((k_muscl<Math, 0, false>), grid, block, 0, s, a);
+    } else {
+        if (resid) hipLaunchKernelGGL((k_muscl<Math, 1, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((k_muscl<Math, 1, false>), grid, block, 0, s, a);
+    }
+    return hipGetLastError();
+}
+
+template <>
+hipError_t launch_ghost<Real>(const HfGhost* tab, int n, const HfBlockDev* blocks, const Real* S, Real* D,
+                              const Real* consts, unsigned long long* gmax, double gamma, const GhostBook& book,
+                              hipStream_t s)
+{
+    const int nb = n > 0 ? (n + 255) / 256 : 1;
+    hipLaunchKernelGGL((k_ghost<Math>), dim3(nb), dim3(256), 0, s, tab, n, blocks, S, D, consts, gmax, gamma, book);
+    return hipGetLastError();
+}
+
+template <>
+hipError_t launch_scan<Real>(const HfBlockDev& bd, const Real* I, const Real* Gh, int interior_only, double gamma,
+                             unsigned long long* out, hipStream_t s)
+{
+    const long long cells = (long long)bd.tx * bd.ty;
+    long long nb = (cells + 255) / 256;
+    if (nb > 4096) nb = 4096;
+    hipLaunchKernelGGL((k_scan<Math>), dim3((unsigned)nb), dim3(256), 0, s, bd, I, Gh, interior_only, gamma, out);
+    return hipGetLastError();
+}
+
+template <>
+hipError_t launch_conserve<Real>(const HfBlockDev& bd, const Real* A, double* part, int nwg, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_conserve<Real>), dim3(nwg), dim3(256), 0, s, bd, A, part);
+    return hipGetLastError();
+}
+
+}  // namespace hf

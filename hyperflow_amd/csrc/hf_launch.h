@@ -1,0 +1,22 @@
+// hf_launch.h -- host-side launchers of the gfx950 kernels (one translation
+// unit per arithmetic type instantiates them).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "hf_common.h"
+
+namespace hf {
+
+
+template <class R>
+hipError_t launch_update(int scheme, int limiter, bool resid, const HfStepArgs<R>& a, hipStream_t s);
+template <class R>
+hipError_t launch_ghost(const HfGhost* tab, int n, const HfBlockDev* blocks, const R* S, R* D, const R* consts,
+                        unsigned long long* gmax, double gamma, const GhostBook& book, hipStream_t s);
+template <class R>
+hipError_t launch_scan(const HfBlockDev& bd, const R* I, const R* Gh, int interior_only, double gamma,
+                       unsigned long long* out, hipStream_t s);
+template <class R>
+hipError_t launch_conserve(const HfBlockDev& bd, const R* A, double* part, int nwg, hipStream_t s);
+
+}  // namespace hf

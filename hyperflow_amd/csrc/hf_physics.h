@@ -1,0 +1,306 @@
+// hf_physics.h -- per-cell and per-face arithmetic of the Euler step for gfx950.
+//
+// Every expression keeps the reference's evaluation order (binary64 with the
+// FP64 policy, compiled with -ffp-contract=off), so the FP64 kernels are
+// bit-identical to the reference CPU path.  Paths below are relative to the
+// reference checkout under src/fvm_2d_cart/.
+//
+// Deliberate, bit-preserving differences from the reference's code shape:
+//  * primitives, E and the sound speed of a state are computed once and
+//    shared by its flux and the HLLC star state (same formula, same operands);
+//  * HLLC evaluates only the star state of the selected branch;
+//  * the limiters evaluate one division (a correctly rounded quotient is
+//    monotone in the numerator, so min(2r/(1+r), 2/(1+r)) = min(2r,2)/(1+r)).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace hf {
+
+// Exact binary64: IEEE division and the correctly rounded square root.
+struct MathF64 {
+    typedef double R;
+    static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+    static __device__ __forceinline__ double sqrt(double x) { return __builtin_sqrt(x); }
+    static __device__ __forceinline__ double fabs(double x) { return __builtin_fabs(x); }
+    static constexpr double tol = 1e-6;
+};
+
+// Bulk FP32: hardware reciprocal and square root (about 1 ulp each).
+struct MathF32 {
+    typedef float R;
+    static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+    static __device__ __forceinline__ float fabs(float x) { return __builtin_fabsf(x); }
+    static constexpr float tol = 1e-6f;
+};
+
+// std::min / std::max (NaN handling of the reference:
+// scheme/riemann/riemann_hllc/riemann_hllc.cpp:146-147, timestep.cpp:51-52).
+template <class R> __device__ __forceinline__ R smin(R a, R b) { return (b < a) ? b : a; }
+template <class R> __device__ __forceinline__ R smax(R a, R b) { return (a < b) ? b : a; }
+
+// A conserved state with its primitives, total energy (re-derived from the
+// primitives) and sound speed.
+template <class R>
+struct State {
+    R U0, U1, U2, U3;   // rho, rho*u, rho*v, E
+    R u, v, p, E, a;
+};
+
+// EulerIdealGasModel::cons_to_prim (model/euler_ideal_gas/euler_ideal_gas.cpp:76-92)
+// and prim_total_energy / prim_internal_energy (:32-46).
+template <class M>
+__device__ __forceinline__ void prims(typename M::R U0, typename M::R U1, typename M::R U2, typename M::R U3,
+                                      typename M::R g1, typename M::R& u, typename M::R& v,
+                                      typename M::R& p, typename M::R& E)
+{
+    typedef typename M::R R;
+    u = M::div(U1, U0);
+    v = M::div(U2, U0);
+    const R q2 = u * u + v * v;
+    p = g1 * (U3 - R(0.5) * U0 * q2);
+    const R e = M::div(p, U0 * g1);
+    E = U0 * (R(0.5) * q2 + e);
+}
+
+// prim_speed_of_sound (euler_ideal_gas.cpp:60-64), also used by
+// calc_wave_speeds_direct_min_max (riemann_hllc.cpp:130-131).
+template <class M>
+__device__ __forceinline__ typename M::R sound(typename M::R gamma, typename M::R p, typename M::R rho)
+{
+    return M::sqrt(M::div(gamma * p, rho));
+}
+
+template <class M>
+__device__ __forceinline__ State<typename M::R> make_state(const typename M::R U[4], typename M::R gamma,
+                                                           typename M::R g1)
+{
+    State<typename M::R> s;
+    s.U0 = U[0]; s.U1 = U[1]; s.U2 = U[2]; s.U3 = U[3];
+    prims<M>(U[0], U[1], U[2], U[3], g1, s.u, s.v, s.p, s.E);
+    s.a = sound<M>(gamma, s.p, U[0]);
+    return s;
+}
+
+// cons_flux_x / cons_flux_y (euler_ideal_gas.cpp:113-149) from precomputed
+// primitives: [rho u, rho u u + p, rho u v, u (E + p)] and the y mirror.
+template <class R, bool X>
+__device__ __forceinline__ void flux(R rho, R u, R v, R p, R E, R F[4])
+{
+    if (X) {
+        F[0] = rho * u;
+        F[1] = rho * u * u + p;
+        F[2] = rho * u * v;
+        F[3] = u * (E + p);
+    } else {
+        F[0] = rho * v;
+        F[1] = rho * u * v;
+        F[2] = rho * v * v + p;
+        F[3] = v * (E + p);
+    }
+}
+
+// HLLCEulerRiemannSolver::calc_hllc_flux (riemann_hllc.cpp:169-212) with the
+// Davis wave speeds (:127-150), S* (:67-97), U*_K (:37-63), F*_K (:153-164).
+// NaN or inconsistent states fall through to a zero flux (:207-210).
+template <class M, bool X>
+__device__ __forceinline__ void hllc(const State<typename M::R>& L, const State<typename M::R>& Rs,
+                                     typename M::R F[4])
+{
+    typedef typename M::R R;
+    const R qL = X ? L.u : L.v, qR = X ? Rs.u : Rs.v;
+    const R S_L = smin(qL - L.a, qR - Rs.a);
+    const R S_R = smax(qL + L.a, qR + Rs.a);
+    const R num = Rs.p - L.p + L.U0 * qL * (S_L - qL) - Rs.U0 * qR * (S_R - qR);
+    const R den = L.U0 * (S_L - qL) - Rs.U0 * (S_R - qR);
+    const R Ss = M::div(num, den);
+
+    const bool c1 = R(0) <= S_L;
+    const bool c2 = (S_L <= R(0)) && (R(0) <= Ss);
+    const bool c3 = (Ss <= R(0)) && (R(0) <= S_R);
+    const bool c4 = R(0) >= S_R;
+    const bool useR = !c1 && !c2 && c3;   // F*_R branch; otherwise the star state of L
+
+    // the side whose star state is needed
+    const R dK = useR ? Rs.U0 : L.U0;
+    const R uK = useR ? Rs.u : L.u;
+    const R vK = useR ? Rs.v : L.v;
+    const R pK = useR ? Rs.p : L.p;
+    const R EK = useR ? Rs.E : L.E;
+    const R SK = useR ? S_R : S_L;
+    const R qK = X ? uK : vK;
+    const R tK = X ? vK : uK;
+    const R coef = dK * M::div(SK - qK, SK - Ss);
+    const R us0 = coef;
+    const R usn = coef * Ss;
+    const R ust = coef * tK;
+    const R us3 = coef * (M::div(EK, dK) + (Ss - qK) * (Ss + M::div(pK, dK * (SK - qK))));
+
+    R FL[4], FR[4];
+    flux<R, X>(L.U0, L.u, L.v, L.p, L.E, FL);
+    flux<R, X>(Rs.U0, Rs.u, Rs.v, Rs.p, Rs.E, FR);
+    const R K0 = useR ? Rs.U0 : L.U0;
+    const R K1 = useR ? Rs.U1 : L.U1;
+    const R K2 = useR ? Rs.U2 : L.U2;
+    const R K3 = useR ? Rs.U3 : L.U3;
+    R FK[4];
+    for (int i = 0; i < 4; i++) FK[i] = useR ? FR[i] : FL[i];
+    R Fs[4];
+    Fs[0] = FK[0] + SK * (us0 - K0);
+    Fs[1] = FK[1] + SK * ((X ? usn : ust) - K1);
+    Fs[2] = FK[2] + SK * ((X ? ust : usn) - K2);
+    Fs[3] = FK[3] + SK * (us3 - K3);
+
+    for (int i = 0; i < 4; i++) {
+        R f = R(0);
+        if (c1) f = FL[i];
+        else if (c2 || c3) f = Fs[i];
+        else if (c4) f = FR[i];
+        F[i] = f;
+    }
+}
+
+// Tolerance clamp of the limiter ratio (scheme/muscl_hancock/muscl_hancock.cpp:8-11,78-94).
+template <class M>
+__device__ __forceinline__ typename M::R clamp_delta(typename M::R d)
+{
+    typedef typename M::R R;
+    const R tol = R(M::tol);
+    if (M::fabs(d) <= tol) return (d == R(0)) ? tol : ((d > R(0)) ? tol : -tol);
+    return d;
+}
+
+// MinBeeLimiter::operator() (scheme/limiter/minbee/minbee.cpp:29-38) and
+// VanLeerLimiter::operator() (scheme/limiter/vanleer/vanleer.cpp:29-36) with
+// omega=0, beta=1 (parser.cpp:231,233); xi_right = 2/(1+r) (limiter.cpp:48-53),
+// whose denominator guard never triggers on the evaluated branches.
+template <class M, int LIM>
+__device__ __forceinline__ typename M::R limiter(typename M::R r)
+{
+    typedef typename M::R R;
+    if (LIM == 0) {
+        const R x = M::div(R(2), R(1) + r);
+        return (r <= R(0)) ? R(0) : ((r <= R(1)) ? r : ((x < R(1)) ? x : R(1)));
+    } else {
+        // min(2r/(1+r), 2/(1+r)) with one division; r = +inf gives inf/inf = NaN
+        // exactly as std::min(NaN, 0) does in the reference.
+        const R inf = __builtin_huge_val();
+        const R num = (r < R(1)) ? R(2) * r : ((r == inf) ? inf : R(2));
+        const R x = M::div(num, R(1) + r);
+        return (r <= R(0)) ? R(0) : x;
+    }
+}
+
+// Limited slope of one conserved variable (muscl_hancock.cpp:137-206):
+// xi(r) * (0.5*(1+omega)*dl + 0.5*(1-omega)*dr) with unclamped deltas.
+template <class M, int LIM>
+__device__ __forceinline__ typename M::R slope(typename M::R um, typename M::R uc, typename M::R up)
+{
+    typedef typename M::R R;
+    const R dl = uc - um;
+    const R dr = up - uc;
+    const R r = M::div(clamp_delta<M>(dl), clamp_delta<M>(dr));
+    return limiter<M, LIM>(r) * (R(0.5) * dl + R(0.5) * dr);
+}
+
+// One direction of MUSCL-Hancock reconstruction for one cell: boundary
+// extrapolation (muscl_hancock.cpp:209-240) and the direction-split half-step
+// predictor (:243-281): lo/hi are the face states (L,R) or (B,T).
+template <class M, int LIM, bool X>
+__device__ __forceinline__ void reconstruct(const typename M::R um[4], const typename M::R uc[4],
+                                            const typename M::R up[4], typename M::R h, typename M::R g1,
+                                            typename M::R lo[4], typename M::R hi[4])
+{
+    typedef typename M::R R;
+    for (int k = 0; k < 4; k++) {
+        const R s = slope<M, LIM>(um[k], uc[k], up[k]);
+        lo[k] = uc[k] - R(0.5) * s;
+        hi[k] = uc[k] + R(0.5) * s;
+    }
+    R ul, vl, pl, El, ur, vr, pr, Er;
+    prims<M>(lo[0], lo[1], lo[2], lo[3], g1, ul, vl, pl, El);
+    prims<M>(hi[0], hi[1], hi[2], hi[3], g1, ur, vr, pr, Er);
+    R Fl[4], Fr[4];
+    flux<R, X>(lo[0], ul, vl, pl, El, Fl);
+    flux<R, X>(hi[0], ur, vr, pr, Er, Fr);
+    for (int k = 0; k < 4; k++) {
+        const R d = Fl[k] - Fr[k];
+        lo[k] += h * d;
+        hi[k] += h * d;
+    }
+}
+
+// Bits of a non-negative double; their unsigned order is the numeric order,
+// which lets wave-speed maxima use 64-bit integer atomics.
+__device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
+__device__ __forceinline__ double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
+
+// Largest |u|+c and |v|+c of a conserved state (the per-cell terms of
+// TimeStep::operator(), timestep.cpp:42-52); NaN terms are skipped by the caller.
+template <class M>
+__device__ __forceinline__ void wave_speeds(typename M::R U0, typename M::R U1, typename M::R U2,
+                                            typename M::R U3, typename M::R gamma, typename M::R g1,
+                                            double& Dx, double& Dy)
+{
+    typedef typename M::R R;
+    const R u = M::div(U1, U0);
+    const R v = M::div(U2, U0);
+    const R p = g1 * (U3 - R(0.5) * U0 * (u * u + v * v));
+    const R c = sound<M>(gamma, p, U0);
+    Dx = (double)(M::fabs(u) + c);
+    Dy = (double)(M::fabs(v) + c);
+}
+
+__device__ __forceinline__ double dmax_nan(double acc, double x) { return (x > acc) ? x : acc; }
+
+// Reduce a per-lane maximum over the wave (NaN-free inputs).
+__device__ __forceinline__ double wave_max(double x)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(x, off, 64);
+        x = (o > x) ? o : x;
+    }
+    return x;
+}
+
+// The step and predictor dt of a block from the device partial maxima
+// (TimeStep::operator(), timestep.cpp:29-68).  For a block the minimum of
+// dx/(|u|+c) over its cells equals dx / max(|u|+c): correctly rounded
+// division is monotone in the divisor.  A block whose terms were all NaN has
+// max 0 and contributes dx/0 = inf, which the 1e16 start value absorbs.
+__device__ __forceinline__ double cfl_eff(double cfl, long long call)
+{
+    return (call <= 10) ? 0.2 * cfl : cfl;   // timestep.cpp:60-65,72,75
+}
+
+template <class R>
+__device__ __forceinline__ void step_dts(const HfStepArgs<R>& a, int block, const HfBlockDev& bd, int scheme_muscl,
+                                         double& dt_step, double& dt_pred, long long& kstep, long long& kpred)
+{
+    const HfProbDev pr = a.probs[bd.problem];
+    kstep = (long long)a.calls[bd.problem] - pr.cps + 1;
+    kpred = kstep + 1 + bd.lb;
+    double dtx = 1e16, dty = 1e16, ptx = 1e16, pty = 1e16;
+    for (int c = pr.first; c < pr.first + pr.nb; c++) {
+        const double cdx = a.blocks[c].dx, cdy = a.blocks[c].dy;
+        const double gx = bitsd(a.gmax[2 * c]), gy = bitsd(a.gmax[2 * c + 1]);
+        const double ox = smax(bitsd(a.imax_old[2 * c]), gx), oy = smax(bitsd(a.imax_old[2 * c + 1]), gy);
+        dtx = smin(dtx, cdx / ox);
+        dty = smin(dty, cdy / oy);
+        if (scheme_muscl && a.dt_mode == 0) {
+            // blocks before this one are already at U^{n+1} (simulation.cpp:106-110)
+            double px = ox, py = oy;
+            if (c < block) {
+                px = smax(bitsd(a.imax_new[2 * c]), gx);
+                py = smax(bitsd(a.imax_new[2 * c + 1]), gy);
+            }
+            ptx = smin(ptx, cdx / px);
+            pty = smin(pty, cdy / py);
+        }
+    }
+    dt_step = cfl_eff(a.cfl, kstep) * smin(dtx, dty);
+    if (scheme_muscl && a.dt_mode == 0) dt_pred = cfl_eff(a.cfl, kpred) * smin(ptx, pty);
+    else dt_pred = dt_step;
+}
+
+}  // namespace hf

@@ -1,0 +1,69 @@
+"""Writes the golden fixtures of tests/golden/ by running the reference's own
+classes (oracle/_ref/ref_harness, built from the reference sources by
+oracle/Makefile).  Run in a checkout where the reference is present:
+
+    python tests/golden/make_golden.py
+
+Each fixture is an .npz with the case JSON (as text), the initial padded
+fields, the fields after `nsteps` steps and every TimeStep result.  Only data
+is stored; no reference source.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import hfo  # noqa: E402
+from hyperflow_amd import case as C  # noqa: E402
+
+
+def cases():
+    out = []
+    out.append(("godunov_quad_24x17_s0", C.quad_case(24, 17, 0, scheme="Godunov", cfl=0.8), None, (1, 100)))
+    out.append(("muscl_minbee_quad_24x17_s1", C.quad_case(24, 17, 1, scheme="MUSCLHancock", limiter="MinBee",
+                                                           cfl=0.5), None, (1, 100)))
+    out.append(("muscl_vanleer_reflective_20x20_s2",
+                C.quad_case(20, 20, 2, scheme="MUSCLHancock", limiter="VanLeer", cfl=0.5, bc="Reflective"),
+                None, (1, 100)))
+    c = C.quad_case(16, 12, 3, scheme="MUSCLHancock", limiter="VanLeer", cfl=0.5)
+    out.append(("muscl_vanleer_perturbed_16x12_s3", c, C.perturbed_fields(c, 3), (1, 100)))
+    out.append(("step_channel_minbee_8", C.step_channel(8, "MinBee", 0.5), None, (1, 100)))
+    g = C.step_channel(6, "MinBee", 0.5)
+    g.scheme, g.limiter = "Godunov", None
+    out.append(("step_channel_godunov_6", g, None, (1, 100)))
+    for p in ("shock_tubes/toro/toro_shock_tube_01_godunov_hllc_100x5.json",
+              "shock_tubes/toro/toro_shock_tube_01_muscl_hancock_vanleer_hllc_100x5.json",
+              "sphericalrp/sphericalrp_muscl_hancock_vanleer_hllc_50x50.json"):
+        src = os.path.join("/root/reference/cases/fvm_2d_cart", p)
+        if os.path.exists(src):
+            out.append(("case_" + os.path.basename(p)[:-5], C.Case.load(src), None, (1, 40)))
+    return out
+
+
+def main():
+    if not hfo.have_reference():
+        sys.exit("oracle/_ref/ref_harness is not built (needs the reference checkout)")
+    for name, case, fields, steps in cases():
+        init = fields if fields is not None else case.initial_fields()
+        data = {"case_json": np.array(json.dumps(case.to_json())), "nblocks": np.array(len(case.blocks))}
+        for b, U in enumerate(init):
+            data[f"init_{b}"] = U
+        for n in steps:
+            f, calls, sdt, _ = hfo.run_reference(case, n, init)
+            for b, U in enumerate(f):
+                data[f"after{n}_{b}"] = U
+            data[f"calls{n}"] = calls
+            data[f"stepdt{n}"] = sdt
+        data["steps"] = np.array(steps)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **data)
+        print("wrote", name)
+
+
+if __name__ == "__main__":
+    main()

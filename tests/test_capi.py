@@ -1,0 +1,81 @@
+"""The C-ABI library: loads, exports every entry point of include/hyperflow.h,
+and validates case descriptors on the host (no GPU work is issued here)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from hyperflow_amd import native
+from hyperflow_amd import case as K
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "hyperflow.h")).read()
+    return sorted(set(re.findall(r"HF_API\s+[\w\s\*]+?\b(hf_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_entry_point():
+    lib = native.lib()
+    names = header_functions()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(native.EXPORTS)
+
+
+def test_struct_layout_matches_header():
+    assert C.sizeof(native.hf_block_desc) == 4 * 8 + 2 * 4 + 4 * 4 + 4 * 4 + 16 * 8 + 2 * 4
+    assert C.sizeof(native.hf_case_desc) == 4 * 4 + 2 * 8 + 2 * 4 + 8
+
+
+def test_version_string():
+    assert "gfx950" in native.version()
+
+
+def _desc(**over):
+    case = K.quad_case(8, 8, 0, scheme="Godunov")
+    arr = native.block_descs([case])
+    d = native.hf_case_desc()
+    d.scheme, d.limiter, d.precision, d.dt_mode = 0, 0, 0, 0
+    d.gamma, d.cfl, d.nblocks, d.tile_rows = 1.4, 0.5, 1, 0
+    d.blocks = C.cast(arr, C.POINTER(native.hf_block_desc))
+    for k, v in over.items():
+        if k.startswith("block_"):
+            setattr(arr[0], k[6:], v)
+        else:
+            setattr(d, k, v)
+    return d, arr
+
+
+@pytest.mark.parametrize("over,msg", [
+    ({"scheme": 7}, "scheme"),
+    ({"scheme": 1, "limiter": 9}, "limiter"),
+    ({"precision": 3}, "precision"),
+    ({"cfl": -1.0}, "cfl"),
+    ({"nblocks": 0}, "no blocks"),
+    ({"block_nx": 0}, "nx"),
+    ({"block_x_right": -1.0}, "extents"),
+    ({"block_problem": 1}, "problems"),
+])
+def test_invalid_descriptors_are_rejected(over, msg):
+    d, arr = _desc(**over)
+    h = C.c_void_p()
+    rc = native.lib().hf_create(C.byref(d), C.byref(h))
+    assert rc == -1
+    assert msg in native.lib().hf_last_error().decode()
+    assert not h.value
+
+
+def test_internal_neighbour_validation():
+    case = K.step_channel(8)
+    arr = native.block_descs([case])
+    arr[1].ny = 9     # shared left/right edge no longer matches
+    d = native.hf_case_desc()
+    d.scheme, d.limiter, d.gamma, d.cfl, d.nblocks = 1, 0, 1.4, 0.5, len(arr)
+    d.blocks = C.cast(arr, C.POINTER(native.hf_block_desc))
+    h = C.c_void_p()
+    assert native.lib().hf_create(C.byref(d), C.byref(h)) == -1
+    assert "Internal" in native.lib().hf_last_error().decode()
