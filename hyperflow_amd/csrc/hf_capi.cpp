@@ -71,7 +71,10 @@ struct hf_ctx {
     std::vector<HfProbDev> probs;
     std::vector<HfItem> items;
     std::vector<int> item_begin, item_count;      // per block
-    std::vector<std::pair<int, int>> rounds;      // item ranges launched together
+    std::vector<std::pair<int, int>> fix_rounds;  // item ranges of the fix-up launches (lb = 1, 2, ..)
+    int main_items = 0;                           // items [0, main_items) cover every block in order
+    int maxnb = 1;
+    int wg_cap = 1024;                            // resident workgroups of an update launch
     int64_t total = 0;                            // elements per state buffer
     void* buf[2] = {nullptr, nullptr};
     int cur = 0;
@@ -88,6 +91,9 @@ struct hf_ctx {
     unsigned long long* d_imax = nullptr;   // [2][nb][2]
     unsigned long long* d_gmax = nullptr;   // [2][nb][2]
     unsigned long long* d_scan = nullptr;   // [nb][2] scan results + [nb][2] scratch maxima
+    unsigned long long* d_imax_fix = nullptr;   // [nb][2]
+    uint32_t* d_fixed = nullptr;                // [nb]
+    uint32_t* d_queue = nullptr;                // [maxnb + 1] work-queue counters
     uint32_t* d_calls = nullptr;
     uint32_t* h_calls_pinned = nullptr;
     hipEvent_t calls_copied = nullptr;   // the last upload of h_calls_pinned has been read
@@ -301,6 +307,10 @@ HfStepArgs<R> step_args(hf_ctx* c, int item0, int nitems, const void* A, const v
     a.A = (const R*)A;
     a.G = (const R*)G;
     a.O = (R*)O;
+    a.nprob = c->np;
+    a.queue = c->d_queue;
+    a.imax_fix = c->d_imax_fix;
+    a.fixed = c->d_fixed;
     a.imax_old = c->imax(c->cur);
     a.imax_new = c->imax(c->cur ^ 1);
     a.gmax = c->gmax(c->cur);
@@ -328,7 +338,8 @@ int launch_update_t(hf_ctx* c, HfStepArgs<R>& a, bool resid, bool timed)
         }
         HF_HIP(hipEventRecord(e0, c->stream));
     }
-    HF_HIP(hf::launch_update<R>(c->scheme, c->limiter, resid, a, c->stream));
+    const int grid = std::max(1, std::min(c->wg_cap, (a.nitems + 3) / 4));
+    HF_HIP(hf::launch_update<R>(c->scheme, c->limiter, resid, a, grid, c->stream));
     if (timed) {
         HF_HIP(hipEventRecord(e1, c->stream));
         c->ev_pending.emplace_back(e0, e1);
@@ -336,21 +347,37 @@ int launch_update_t(hf_ctx* c, HfStepArgs<R>& a, bool resid, bool timed)
     return HF_OK;
 }
 
+// One persistent update launch over items [item0, item0+nitems), taking
+// tickets from queue slot qslot (zeroed before the launch runs).
 int launch_update(hf_ctx* c, int item0, int nitems, const void* A, const void* G, void* O, bool resid,
-                  bool fixed, double dt_step, double dt_pred, bool timed)
+                  bool fixed, double dt_step, double dt_pred, bool timed, int qslot, int fix_lb)
 {
     if (c->prec == HF_FP64) {
         HfStepArgs<double> a = step_args<double>(c, item0, nitems, A, G, O);
         a.use_fixed = fixed;
         a.fixed_step = dt_step;
         a.fixed_pred = dt_pred;
+        a.queue = c->d_queue + qslot;
+        a.fix_lb = fix_lb;
         return launch_update_t<double>(c, a, resid, timed);
     }
     HfStepArgs<float> a = step_args<float>(c, item0, nitems, A, G, O);
     a.use_fixed = fixed;
     a.fixed_step = dt_step;
     a.fixed_pred = dt_pred;
+    a.queue = c->d_queue + qslot;
+    a.fix_lb = fix_lb;
     return launch_update_t<float>(c, a, resid, timed);
+}
+
+// A launch outside the device-resident loop (class-level API): its queue
+// slot is cleared from the host first.
+int launch_update_host(hf_ctx* c, int item0, int nitems, const void* A, const void* G, void* O, bool resid,
+                       double dt_step, double dt_pred, bool timed)
+{
+    const int slot = c->maxnb;
+    HF_HIP(hipMemsetAsync(c->d_queue + slot, 0, sizeof(uint32_t), c->stream));
+    return launch_update(c, item0, nitems, A, G, O, resid, true, dt_step, dt_pred, timed, slot, 0);
 }
 
 int launch_ghost(hf_ctx* c, const HfGhost* tab, int n, const void* S, void* D, unsigned long long* gmax,
@@ -424,6 +451,8 @@ int ensure_partials(hf_ctx* c)
     if (c->partials_valid) return HF_OK;
     HF_HIP(hipMemsetAsync(c->imax(c->cur), 0, sizeof(unsigned long long) * 2 * c->nb, c->stream));
     HF_HIP(hipMemsetAsync(c->gmax(c->cur), 0, sizeof(unsigned long long) * 2 * c->nb, c->stream));
+    HF_HIP(hipMemsetAsync(c->d_imax_fix, 0, sizeof(unsigned long long) * 2 * c->nb, c->stream));
+    HF_HIP(hipMemsetAsync(c->d_fixed, 0, sizeof(uint32_t) * c->nb, c->stream));
     for (int b = 0; b < c->nb; b++) {
         int rc = launch_scan(c, b, c->buf[c->cur], c->buf[c->cur], 1, c->imax(c->cur) + 2 * b);
         if (rc) return rc;
@@ -448,10 +477,19 @@ int enqueue_steps(hf_ctx* c, int nsteps, bool timed)
         bk.clear0 = c->gmax(k ^ 1);
         bk.clear1 = c->imax(k ^ 1);
         bk.nclear = 2 * c->nb;
+        bk.queue = c->d_queue;
+        bk.nqueue = c->maxnb;
+        bk.nblocks = c->nb;
+        bk.imax_prev = c->imax(k);
+        bk.imax_fix = c->d_imax_fix;
+        bk.fixed = c->d_fixed;
         int rc = launch_ghost(c, c->d_fill, c->nfill, A, B, c->gmax(k), bk);
         if (rc) return rc;
-        for (const auto& r : c->rounds) {
-            rc = launch_update(c, r.first, r.second, A, B, B, false, false, 0.0, 0.0, timed);
+        rc = launch_update(c, 0, c->main_items, A, B, B, false, false, 0.0, 0.0, timed, 0, 0);
+        if (rc) return rc;
+        for (size_t r = 0; r < c->fix_rounds.size(); r++) {
+            rc = launch_update(c, c->fix_rounds[r].first, c->fix_rounds[r].second, A, B, B, false, false, 0.0, 0.0,
+                               false, (int)r + 1, (int)r + 1);
             if (rc) return rc;
         }
         c->cur ^= 1;
@@ -566,33 +604,33 @@ int hf_create(const hf_case_desc* desc, hf_ctx** out)
     for (auto& P : c->probs) P.cps = c->scheme == HF_MUSCL_HANCOCK ? 1 + P.nb : 1;
     c->calls.assign(c->np, 0);
 
-    // work items: rounds of one block per problem (compat MUSCL) or one round
+    // work items: every block in order (the main launch of a step), then,
+    // for compat MUSCL problems of several blocks, the lb-th blocks of all
+    // problems for each fix-up launch lb = 1, 2, ..
     const int W = c->g == 2 ? 60 : 62;
     const int rows = desc->tile_rows > 0 ? desc->tile_rows : 16;
-    int maxnb = 0;
-    for (auto& P : c->probs) maxnb = std::max(maxnb, P.nb);
-    const bool per_block_rounds = c->scheme == HF_MUSCL_HANCOCK && c->dt_mode == HF_DT_COMPAT && maxnb > 1;
-    const int nrounds = per_block_rounds ? maxnb : 1;
+    for (auto& P : c->probs) c->maxnb = std::max(c->maxnb, P.nb);
     c->item_begin.assign(c->nb, 0);
     c->item_count.assign(c->nb, 0);
-    for (int r = 0; r < nrounds; r++) {
-        const int first = (int)c->items.size();
-        for (int p = 0; p < c->np; p++) {
-            const HfProbDev& P = c->probs[p];
-            for (int lb = 0; lb < P.nb; lb++) {
-                if (per_block_rounds && lb != r) continue;
-                const int b = P.first + lb;
-                const HfBlockDev& B = c->blocks[b];
-                c->item_begin[b] = (int)c->items.size();
-                const int strips = (B.nx + W - 1) / W;
-                for (int y = 0; y < B.ny; y += rows)
-                    for (int s = 0; s < strips; s++)
-                        c->items.push_back(HfItem{b, s * W, c->g + y, std::min(rows, B.ny - y)});
-                c->item_count[b] = (int)c->items.size() - c->item_begin[b];
-            }
-        }
-        c->rounds.emplace_back(first, (int)c->items.size() - first);
+    auto add_block_items = [&](int b) {
+        const HfBlockDev& B = c->blocks[b];
+        const int strips = (B.nx + W - 1) / W;
+        for (int y = 0; y < B.ny; y += rows)
+            for (int s = 0; s < strips; s++) c->items.push_back(HfItem{b, s * W, c->g + y, std::min(rows, B.ny - y)});
+    };
+    for (int b = 0; b < c->nb; b++) {
+        c->item_begin[b] = (int)c->items.size();
+        add_block_items(b);
+        c->item_count[b] = (int)c->items.size() - c->item_begin[b];
     }
+    c->main_items = (int)c->items.size();
+    if (c->scheme == HF_MUSCL_HANCOCK && c->dt_mode == HF_DT_COMPAT)
+        for (int lb = 1; lb < c->maxnb; lb++) {
+            const int first = (int)c->items.size();
+            for (const auto& P : c->probs)
+                if (P.nb > lb) add_block_items(P.first + lb);
+            c->fix_rounds.emplace_back(first, (int)c->items.size() - first);
+        }
 
     std::vector<HfGhost> fill, copy;
     build_tables(*c, fill, copy);
@@ -646,6 +684,20 @@ int hf_create(const hf_case_desc* desc, hf_ctx** out)
     HF_HIPC(hipMalloc(&C->d_scan, sizeof(unsigned long long) * 4 * C->nb));
     HF_HIPC(hipMemset(C->d_imax, 0, sizeof(unsigned long long) * 4 * C->nb));
     HF_HIPC(hipMemset(C->d_gmax, 0, sizeof(unsigned long long) * 4 * C->nb));
+    HF_HIPC(hipMalloc(&C->d_imax_fix, sizeof(unsigned long long) * 2 * C->nb));
+    HF_HIPC(hipMemset(C->d_imax_fix, 0, sizeof(unsigned long long) * 2 * C->nb));
+    HF_HIPC(hipMalloc(&C->d_fixed, sizeof(uint32_t) * C->nb));
+    HF_HIPC(hipMemset(C->d_fixed, 0, sizeof(uint32_t) * C->nb));
+    HF_HIPC(hipMalloc(&C->d_queue, sizeof(uint32_t) * (C->maxnb + 1)));
+    HF_HIPC(hipMemset(C->d_queue, 0, sizeof(uint32_t) * (C->maxnb + 1)));
+    {
+        int dev = 0, cus = 0;
+        HF_HIPC(hipGetDevice(&dev));
+        HF_HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        const int per_cu = C->prec == HF_FP64 ? hf::update_wgs_per_cu<double>(C->scheme, C->limiter, false)
+                                              : hf::update_wgs_per_cu<float>(C->scheme, C->limiter, false);
+        C->wg_cap = std::max(1, cus * per_cu);
+    }
     HF_HIPC(hipMalloc(&C->d_calls, sizeof(uint32_t) * C->np));
     HF_HIPC(hipMemset(C->d_calls, 0, sizeof(uint32_t) * C->np));
     HF_HIPC(hipHostMalloc((void**)&C->h_calls_pinned, sizeof(uint32_t) * C->np, hipHostMallocDefault));
@@ -669,7 +721,8 @@ void hf_destroy(hf_ctx* c)
     for (auto& e : c->ev_free) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
     for (void* p : {c->buf[0], c->buf[1], (void*)c->d_blocks, (void*)c->d_probs, (void*)c->d_items, (void*)c->d_fill,
                     (void*)c->d_copy, c->d_consts, (void*)c->d_imax, (void*)c->d_gmax, (void*)c->d_scan,
-                    (void*)c->d_calls, (void*)c->d_hist, (void*)c->d_part, c->d_resid})
+                    (void*)c->d_calls, (void*)c->d_hist, (void*)c->d_part, c->d_resid, (void*)c->d_imax_fix,
+                    (void*)c->d_fixed, (void*)c->d_queue})
         if (p) hipFree(p);
     if (c->h_calls_pinned) hipHostFree(c->h_calls_pinned);
     if (c->calls_copied) hipEventDestroy(c->calls_copied);
@@ -749,7 +802,7 @@ int hf_scheme_residual(hf_ctx* c, int b, double* L, size_t n)
         if (rc) return rc;
     }
     HF_HIP(hipMemsetAsync(c->d_resid, 0, (size_t)c->resid_elems * c->esize, c->stream));
-    int rc = launch_update(c, c->item_begin[b], c->item_count[b], A, A, c->d_resid, true, true, 0.0, dt_pred, false);
+    int rc = launch_update_host(c, c->item_begin[b], c->item_count[b], A, A, c->d_resid, true, 0.0, dt_pred, false);
     if (rc) return rc;
     return download_block(c, b, c->d_resid, 0, L);
 }
@@ -778,8 +831,8 @@ int hf_advance(hf_ctx* c, const double* dt)
                         HF_HIP(hipMemcpy(c->d_hist + (k - 1), &dt_pred, sizeof(double), hipMemcpyHostToDevice));
                 }
             }
-            int rc = launch_update(c, c->item_begin[q], c->item_count[q], A, B, B, false, true, dt[p], dt_pred,
-                                   c->profile);
+            int rc = launch_update_host(c, c->item_begin[q], c->item_count[q], A, B, B, false, dt[p], dt_pred,
+                                        c->profile);
             if (rc) return rc;
         }
     }

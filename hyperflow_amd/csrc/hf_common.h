@@ -47,6 +47,7 @@ struct HfStepArgs {
     const HfProbDev* probs;
     const HfItem* items;
     int32_t item0, nitems;
+    int32_t nprob, pad0;
     const R* A;   // interior source (U^n)
     const R* G;   // ghost source (filled ghosts)
     R* O;         // destination: U^{n+1} interior, or the residual L
@@ -59,8 +60,11 @@ struct HfStepArgs {
     int32_t dt_mode;                     // 0 compat, 1 uniform
     double gamma, cfl;
     int32_t use_fixed;                   // 1: dt_step/dt_pred below instead of the device partials
-    int32_t pad;
+    int32_t fix_lb;                      // >0: fix-up launch for the fix_lb-th block of each problem
     double fixed_step, fixed_pred;
+    uint32_t* queue;                     // work-queue counter of this launch (zero at launch)
+    unsigned long long* imax_fix;        // [nblocks][2] maxima of blocks recomputed by a fix-up launch
+    uint32_t* fixed;                     // [nblocks] block was recomputed by a fix-up launch this step
 };
 
 // Step bookkeeping done by the first workgroup of a ghost-fill launch.
@@ -72,4 +76,11 @@ struct GhostBook {
     unsigned long long* clear0;   // arrays to zero (next step's partials)
     unsigned long long* clear1;
     int32_t nclear;               // elements in each
+    uint32_t* queue;              // work-queue counters of this step's update launches
+    int32_t nqueue;
+    int32_t nblocks;
+    // fold the previous step's fix-ups into its maxima, then clear them
+    unsigned long long* imax_prev;   // maxima of the state this step starts from
+    unsigned long long* imax_fix;
+    uint32_t* fixed;
 };

@@ -30,10 +30,44 @@ __device__ __forceinline__ void load_cell(const R* __restrict__ base, int64_t pl
     u[3] = base[3 * plane];
 }
 
-template <class R>
-__device__ __forceinline__ R shfl_up1(R x) { return __shfl_up(x, 1, 64); }
-template <class R>
-__device__ __forceinline__ R shfl_dn1(R x) { return __shfl_down(x, 1, 64); }
+// Whole-wave lane shifts by one lane through DPP (wave_shr:1 / wave_shl:1):
+// lane l receives lane l-1 (up) or lane l+1 (down); the end lanes keep their
+// own value.  No LDS round trip, unlike a bpermute shuffle.
+__device__ __forceinline__ int dpp_up(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ int dpp_dn(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ double shfl_up1(double x)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = dpp_up((int)(b & 0xffffffffll)), hi = dpp_up((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double shfl_dn1(double x)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = dpp_dn((int)(b & 0xffffffffll)), hi = dpp_dn((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float shfl_up1(float x) { return __int_as_float(dpp_up(__float_as_int(x))); }
+__device__ __forceinline__ float shfl_dn1(float x) { return __int_as_float(dpp_dn(__float_as_int(x))); }
+
+// Next work item of a persistent launch.  Every lane executes the atomic
+// (only lane 0 adds 1) so that no lane-divergent region surrounds the loop's
+// exit test; lane 0's old value is the wave's ticket.  A lane-0-only branch
+// here let the compiler's control-flow structurizer keep lanes 1..63 in the
+// loop after lane 0 left it.
+__device__ __forceinline__ unsigned next_ticket(uint32_t* queue, int lane)
+{
+    const unsigned t = atomicAdd(queue, lane == 0 ? 1u : 0u);
+    return __builtin_amdgcn_readfirstlane(t);
+}
+
+// Per-wave maxima into the block's slots; every lane holds the reduced value,
+// so every lane may issue the (idempotent) atomic -- no divergent region.
+__device__ __forceinline__ void store_maxima(unsigned long long* out, int block, double mx, double my)
+{
+    atomicMax(&out[2 * block], dbits(mx));
+    atomicMax(&out[2 * block + 1], dbits(my));
+}
 
 // Load row y of this lane's column: ghost rows/columns come from the ghost
 // source, interior cells from the interior source.
@@ -50,13 +84,11 @@ __device__ __forceinline__ void load_row(const R* __restrict__ Acol, const R* __
 // fused with ForwardEulerODESolver (scheme/ode/forward_euler/forward_euler.cpp:25-30).
 // Output columns per wave: 60 (lanes 2..61); lanes 0,1,62,63 are halo.
 template <class M, int LIM, bool RESID>
-__global__ __launch_bounds__(256) void k_muscl(const HfStepArgs<typename M::R> a)
+__device__ __forceinline__ void muscl_item(const HfStepArgs<typename M::R>& a, const HfItem it, const int lane,
+                                           const double dt_step, const double dt_pred,
+                                           unsigned long long* __restrict__ imax_out)
 {
     typedef typename M::R R;
-    const int lane = threadIdx.x & 63;
-    const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (w >= a.nitems) return;
-    const HfItem it = a.items[a.item0 + w];
 
     const HfBlockDev bd = a.blocks[it.block];
     const int xraw = it.x0 + lane;
@@ -70,19 +102,6 @@ __global__ __launch_bounds__(256) void k_muscl(const HfStepArgs<typename M::R> a
 #define ROW(y, u) load_row<R>(Acol, Gcol, gcol, (y), 2, ty, pitch, plane, (u))
     const bool out = valid && lane >= 2 && lane < 62 && xraw >= 2 && xraw < bd.tx - 2;
 
-    double dt_step, dt_pred;
-    long long kstep, kpred;
-    if (a.use_fixed) {
-        dt_step = a.fixed_step;
-        dt_pred = a.fixed_pred;
-        kstep = kpred = 0;
-    } else {
-        step_dts<R>(a, it.block, bd, 1, dt_step, dt_pred, kstep, kpred);
-        if (bd.problem == 0 && it.x0 == 0 && it.y0 == bd.g && lane == 0) {
-            if (bd.lb == 0 && kstep >= 1 && kstep <= a.hist_cap) a.hist[kstep - 1] = dt_step;
-            if (kpred >= 1 && kpred <= a.hist_cap) a.hist[kpred - 1] = dt_pred;
-        }
-    }
     const R hx = R((0.5 * dt_pred) / bd.dx);     // half_dt_dx, muscl_hancock.cpp:248
     const R hy = R((0.5 * dt_pred) / bd.dy);
     const R rdx = R(1.0 / bd.dx);                  // one_dx, muscl_hancock.cpp:352
@@ -156,15 +175,64 @@ __global__ __launch_bounds__(256) void k_muscl(const HfStepArgs<typename M::R> a
         }
         for (int k = 0; k < 4; k++) { Gs[k] = Gn[k]; Tc[k] = Tn[k]; uc[k] = up[k]; up[k] = un[k]; }
     }
-    if (!RESID) {
-        mx = wave_max(mx);
-        my = wave_max(my);
-        if (lane == 0) {
-            atomicMax(&a.imax_new[2 * it.block], dbits(mx));
-            atomicMax(&a.imax_new[2 * it.block + 1], dbits(my));
-        }
-    }
+    if (!RESID) store_maxima(imax_out, it.block, wave_max(mx), wave_max(my));
 #undef ROW
+}
+
+// Persistent MUSCL-Hancock launch: waves take work items from a queue until
+// it is empty.  The main launch of a step uses the speculative predictor dt
+// (every block at U^n); a fix-up launch (fix_lb > 0) recomputes the fix_lb-th
+// block of a problem only when the reference's dt for it -- earlier blocks
+// already at U^{n+1} -- differs, so results equal the reference's sequential
+// sweep bit for bit.
+template <class M, int LIM, bool RESID>
+__global__ __launch_bounds__(256) void k_muscl(const HfStepArgs<typename M::R> a)
+{
+    typedef typename M::R R;
+    const int lane = threadIdx.x & 63;
+    const bool fix = a.fix_lb > 0;
+    if (fix) {
+        // leave at once unless some problem's fix_lb-th block needs recomputing
+        bool any = false;
+        for (int p = 0; p < a.nprob; p++) {
+            const HfProbDev pr = a.probs[p];
+            if (pr.nb <= a.fix_lb) continue;
+            const int b = pr.first + a.fix_lb;
+            const HfBlockDev bd = a.blocks[b];
+            double s0, sp, t0, tp;
+            long long k0, k1;
+            step_dts<R>(a, b, bd, true, false, s0, sp, k0, k1);
+            step_dts<R>(a, b, bd, true, true, t0, tp, k0, k1);
+            any = any || (sp != tp);
+        }
+        if (!any) return;
+    }
+    for (;;) {
+        const unsigned idx = next_ticket(a.queue, lane);
+        if (idx >= (unsigned)a.nitems) break;
+        const HfItem it = a.items[a.item0 + idx];
+        double dt_step, dt_pred;
+        if (a.use_fixed) {
+            dt_step = a.fixed_step;
+            dt_pred = a.fixed_pred;
+        } else {
+            const HfBlockDev bd = a.blocks[it.block];
+            long long kstep, kpred;
+            step_dts<R>(a, it.block, bd, true, fix, dt_step, dt_pred, kstep, kpred);
+            if (fix) {
+                double s0, sp;
+                long long k0, k1;
+                step_dts<R>(a, it.block, bd, true, false, s0, sp, k0, k1);
+                if (sp == dt_pred) continue;          // main launch already used this dt
+                a.fixed[it.block] = 1u;
+            }
+            if (bd.problem == 0 && it.x0 == 0 && it.y0 == bd.g) {   // same value from every lane
+                if (!fix && bd.lb == 0 && kstep >= 1 && kstep <= a.hist_cap) a.hist[kstep - 1] = dt_step;
+                if (kpred >= 1 && kpred <= a.hist_cap) a.hist[kpred - 1] = dt_pred;
+            }
+        }
+        muscl_item<M, LIM, RESID>(a, it, lane, dt_step, dt_pred, fix ? a.imax_fix : a.imax_new);
+    }
 }
 
 // ---------------------------------------------------------------- Godunov --
@@ -172,13 +240,10 @@ __global__ __launch_bounds__(256) void k_muscl(const HfStepArgs<typename M::R> a
 // the forward Euler update.  Each cell's primitives and sound speed are
 // computed once and shared by its four faces.  Output columns per wave: 62.
 template <class M, bool RESID>
-__global__ __launch_bounds__(256) void k_godunov(const HfStepArgs<typename M::R> a)
+__device__ __forceinline__ void godunov_item(const HfStepArgs<typename M::R>& a, const HfItem it, const int lane,
+                                             const double dt_step, unsigned long long* __restrict__ imax_out)
 {
     typedef typename M::R R;
-    const int lane = threadIdx.x & 63;
-    const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (w >= a.nitems) return;
-    const HfItem it = a.items[a.item0 + w];
 
     const HfBlockDev bd = a.blocks[it.block];
     const int xraw = it.x0 + lane;
@@ -192,17 +257,6 @@ __global__ __launch_bounds__(256) void k_godunov(const HfStepArgs<typename M::R>
 #define ROW(y, u) load_row<R>(Acol, Gcol, gcol, (y), 1, ty, pitch, plane, (u))
     const bool out = valid && lane >= 1 && lane < 63 && xraw >= 1 && xraw < bd.tx - 1;
 
-    double dt_step, dt_pred;
-    long long kstep, kpred;
-    if (a.use_fixed) {
-        dt_step = a.fixed_step;
-        kstep = 0;
-    } else {
-        step_dts<R>(a, it.block, bd, 0, dt_step, dt_pred, kstep, kpred);
-        if (bd.problem == 0 && bd.lb == 0 && it.x0 == 0 && it.y0 == bd.g && lane == 0 &&
-            kstep >= 1 && kstep <= a.hist_cap)
-            a.hist[kstep - 1] = dt_step;
-    }
     const R rdx = R(1.0 / bd.dx);                  // godunov.cpp:97-98
     const R rdy = R(1.0 / bd.dy);
     const R dtR = R(dt_step);
@@ -253,15 +307,33 @@ __global__ __launch_bounds__(256) void k_godunov(const HfStepArgs<typename M::R>
         for (int k = 0; k < 4; k++) Gs[k] = Gn[k];
         sc = sn;
     }
-    if (!RESID) {
-        mx = wave_max(mx);
-        my = wave_max(my);
-        if (lane == 0) {
-            atomicMax(&a.imax_new[2 * it.block], dbits(mx));
-            atomicMax(&a.imax_new[2 * it.block + 1], dbits(my));
-        }
-    }
+    if (!RESID) store_maxima(imax_out, it.block, wave_max(mx), wave_max(my));
 #undef ROW
+}
+
+// Persistent Godunov launch (no predictor, so never a fix-up).
+template <class M, bool RESID>
+__global__ __launch_bounds__(256) void k_godunov(const HfStepArgs<typename M::R> a)
+{
+    typedef typename M::R R;
+    const int lane = threadIdx.x & 63;
+    for (;;) {
+        const unsigned idx = next_ticket(a.queue, lane);
+        if (idx >= (unsigned)a.nitems) break;
+        const HfItem it = a.items[a.item0 + idx];
+        double dt_step, dt_pred;
+        if (a.use_fixed) {
+            dt_step = a.fixed_step;
+        } else {
+            const HfBlockDev bd = a.blocks[it.block];
+            long long kstep, kpred;
+            step_dts<R>(a, it.block, bd, false, false, dt_step, dt_pred, kstep, kpred);
+            if (bd.problem == 0 && bd.lb == 0 && it.x0 == 0 && it.y0 == bd.g && kstep >= 1 &&
+                kstep <= a.hist_cap)
+                a.hist[kstep - 1] = dt_step;
+        }
+        godunov_item<M, RESID>(a, it, lane, dt_step, a.imax_new);
+    }
 }
 
 // ------------------------------------------------------------- ghost fill --
@@ -287,6 +359,16 @@ __global__ __launch_bounds__(256) void k_ghost(const HfGhost* __restrict__ tab, 
         for (int i = threadIdx.x; i < book.nclear; i += blockDim.x) {
             if (book.clear0) book.clear0[i] = 0ull;
             if (book.clear1) book.clear1[i] = 0ull;
+        }
+        for (int i = threadIdx.x; i < book.nqueue; i += blockDim.x) book.queue[i] = 0u;
+        for (int b = threadIdx.x; b < book.nblocks && book.fixed; b += blockDim.x) {
+            if (book.fixed[b]) {
+                book.imax_prev[2 * b] = book.imax_fix[2 * b];
+                book.imax_prev[2 * b + 1] = book.imax_fix[2 * b + 1];
+            }
+            book.fixed[b] = 0u;
+            book.imax_fix[2 * b] = 0ull;
+            book.imax_fix[2 * b + 1] = 0ull;
         }
     }
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;

@@ -10,10 +10,10 @@ typedef HF_MATH Math;
 typedef Math::R Real;
 
 template <>
-hipError_t launch_update<Real>(int scheme, int limiter, bool resid, const HfStepArgs<Real>& a, hipStream_t s)
+hipError_t launch_update<Real>(int scheme, int limiter, bool resid, const HfStepArgs<Real>& a, int nwg, hipStream_t s)
 {
     if (a.nitems <= 0) return hipSuccess;
-    const dim3 block(256), grid((a.nitems + 3) / 4);
+    const dim3 block(256), grid(nwg);
     if (scheme == 0) {
         if (resid) hipLaunchKernelGGL((k_godunov<Math, true>), grid, block, 0, s, a);
         else hipLaunchKernelGGL((k_godunov<Math, false>), grid, block, 0, s, a);
@@ -25,6 +25,24 @@ hipError_t launch_update<Real>(int scheme, int limiter, bool resid, const HfStep
         else hipLaunchKernelGGL((k_muscl<Math, 1, false>), grid, block, 0, s, a);
     }
     return hipGetLastError();
+}
+
+namespace {
+template <class K>
+int occupancy(K kernel)
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, 256, 0) != hipSuccess || n < 1) n = 1;
+    return n;
+}
+}  // namespace
+
+template <>
+int update_wgs_per_cu<Real>(int scheme, int limiter, bool resid)
+{
+    if (scheme == 0) return resid ? occupancy(k_godunov<Math, true>) : occupancy(k_godunov<Math, false>);
+    if (limiter == 0) return resid ? occupancy(k_muscl<Math, 0, true>) : occupancy(k_muscl<Math, 0, false>);
+    return resid ? occupancy(k_muscl<Math, 1, true>) : occupancy(k_muscl<Math, 1, false>);
 }
 
 template <>

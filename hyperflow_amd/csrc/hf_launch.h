@@ -9,7 +9,10 @@ namespace hf {
 
 
 template <class R>
-hipError_t launch_update(int scheme, int limiter, bool resid, const HfStepArgs<R>& a, hipStream_t s);
+hipError_t launch_update(int scheme, int limiter, bool resid, const HfStepArgs<R>& a, int grid, hipStream_t s);
+// Workgroups (256 threads) of an update kernel that fit on one CU at once.
+template <class R>
+int update_wgs_per_cu(int scheme, int limiter, bool resid);
 template <class R>
 hipError_t launch_ghost(const HfGhost* tab, int n, const HfBlockDev* blocks, const R* S, R* D, const R* consts,
                         unsigned long long* gmax, double gamma, const GhostBook& book, hipStream_t s);

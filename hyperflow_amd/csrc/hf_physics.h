@@ -273,34 +273,49 @@ __device__ __forceinline__ double cfl_eff(double cfl, long long call)
     return (call <= 10) ? 0.2 * cfl : cfl;   // timestep.cpp:60-65,72,75
 }
 
+// Largest wave speeds of the new interior of block c after the step's
+// fix-up launches so far.
 template <class R>
-__device__ __forceinline__ void step_dts(const HfStepArgs<R>& a, int block, const HfBlockDev& bd, int scheme_muscl,
-                                         double& dt_step, double& dt_pred, long long& kstep, long long& kpred)
+__device__ __forceinline__ double new_max(const HfStepArgs<R>& a, int c, int axis)
+{
+    return bitsd(a.fixed[c] ? a.imax_fix[2 * c + axis] : a.imax_new[2 * c + axis]);
+}
+
+// Step dt and predictor dt of a block.  The predictor dt follows the
+// reference's compat semantics when final_new is set: blocks of the problem
+// before this one are already at U^{n+1} (simulation.cpp:106-110 with
+// muscl_hancock.cpp:246).  Without final_new every block is taken at U^n,
+// which is the speculative value the main launch uses; a fix-up launch
+// recomputes a block whenever the two differ.
+template <class R>
+__device__ __forceinline__ void step_dts(const HfStepArgs<R>& a, int block, const HfBlockDev& bd, bool muscl,
+                                         bool final_new, double& dt_step, double& dt_pred, long long& kstep,
+                                         long long& kpred)
 {
     const HfProbDev pr = a.probs[bd.problem];
     kstep = (long long)a.calls[bd.problem] - pr.cps + 1;
     kpred = kstep + 1 + bd.lb;
     double dtx = 1e16, dty = 1e16, ptx = 1e16, pty = 1e16;
+    const bool compat = muscl && a.dt_mode == 0;
     for (int c = pr.first; c < pr.first + pr.nb; c++) {
         const double cdx = a.blocks[c].dx, cdy = a.blocks[c].dy;
         const double gx = bitsd(a.gmax[2 * c]), gy = bitsd(a.gmax[2 * c + 1]);
         const double ox = smax(bitsd(a.imax_old[2 * c]), gx), oy = smax(bitsd(a.imax_old[2 * c + 1]), gy);
-        dtx = smin(dtx, cdx / ox);
-        dty = smin(dty, cdy / oy);
-        if (scheme_muscl && a.dt_mode == 0) {
-            // blocks before this one are already at U^{n+1} (simulation.cpp:106-110)
-            double px = ox, py = oy;
-            if (c < block) {
-                px = smax(bitsd(a.imax_new[2 * c]), gx);
-                py = smax(bitsd(a.imax_new[2 * c + 1]), gy);
+        const double sx = cdx / ox, sy = cdy / oy;
+        dtx = smin(dtx, sx);
+        dty = smin(dty, sy);
+        if (compat) {
+            if (final_new && c < block) {
+                ptx = smin(ptx, cdx / smax(new_max(a, c, 0), gx));
+                pty = smin(pty, cdy / smax(new_max(a, c, 1), gy));
+            } else {
+                ptx = smin(ptx, sx);
+                pty = smin(pty, sy);
             }
-            ptx = smin(ptx, cdx / px);
-            pty = smin(pty, cdy / py);
         }
     }
     dt_step = cfl_eff(a.cfl, kstep) * smin(dtx, dty);
-    if (scheme_muscl && a.dt_mode == 0) dt_pred = cfl_eff(a.cfl, kpred) * smin(ptx, pty);
-    else dt_pred = dt_step;
+    dt_pred = compat ? cfl_eff(a.cfl, kpred) * smin(ptx, pty) : dt_step;
 }
 
 }  // namespace hf

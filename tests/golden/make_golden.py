@@ -23,6 +23,25 @@ import hfo  # noqa: E402
 from hyperflow_amd import case as C  # noqa: E402
 
 
+def blast_row(n, limiter):
+    """Three blocks in a row; a pressure blast in block 0 whose wave speeds
+    grow during the first steps, so the compat predictor dt of blocks 1 and 2
+    differs from the one computed before block 0 is updated."""
+    c = C.quad_case(3 * n, n, 0, scheme="MUSCLHancock", limiter=limiter, cfl=0.5,
+                    extents=(0.0, 3.0, 0.0, 1.0))
+    blocks = []
+    for k in range(3):
+        bnd = {"bottom": {"type": "Reflective"}, "top": {"type": "Reflective"},
+               "left": {"type": "Transmissive"} if k == 0 else {"type": "Internal", "block": k - 1},
+               "right": {"type": "Internal", "block": k + 1} if k < 2 else {"type": "Transmissive"}}
+        blocks.append(C.Block(float(k), float(k + 1), 0.0, 1.0, n, n, bnd))
+    c.blocks = blocks
+    c.initial_condition = {"type": "SphericalRiemannProblem", "radius": 0.3, "x_origin": 0.5, "y_origin": 0.5,
+                           "variables": "Primitive", "init_sphere_state": [1.0, 0.0, 0.0, 100.0],
+                           "init_ext_state": [1.0, 0.0, 0.0, 1.0]}
+    return c
+
+
 def cases():
     out = []
     out.append(("godunov_quad_24x17_s0", C.quad_case(24, 17, 0, scheme="Godunov", cfl=0.8), None, (1, 100)))
@@ -37,6 +56,8 @@ def cases():
     g = C.step_channel(6, "MinBee", 0.5)
     g.scheme, g.limiter = "Godunov", None
     out.append(("step_channel_godunov_6", g, None, (1, 100)))
+    out.append(("blast_row_minbee_3x12", blast_row(12, "MinBee"), None, (1, 60)))
+    out.append(("blast_row_vanleer_3x10", blast_row(10, "VanLeer"), None, (1, 60)))
     for p in ("shock_tubes/toro/toro_shock_tube_01_godunov_hllc_100x5.json",
               "shock_tubes/toro/toro_shock_tube_01_muscl_hancock_vanleer_hllc_100x5.json",
               "sphericalrp/sphericalrp_muscl_hancock_vanleer_hllc_50x50.json"):
