@@ -72,7 +72,7 @@ struct hf_ctx {
     std::vector<HfItem> items;
     std::vector<int> item_begin, item_count;      // per block
     std::vector<std::pair<int, int>> fix_rounds;  // item ranges of the fix-up launches (lb = 1, 2, ..)
-    int main_items = 0;                           // items [0, main_items) cover every block in order
+    int main_begin = 0, main_items = 0;           // item range of a step's main launch
     int maxnb = 1;
     int wg_cap = 1024;                            // resident workgroups of an update launch
     int64_t total = 0;                            // elements per state buffer
@@ -94,6 +94,8 @@ struct hf_ctx {
     unsigned long long* d_imax_fix = nullptr;   // [nb][2]
     uint32_t* d_fixed = nullptr;                // [nb]
     uint32_t* d_queue = nullptr;                // [maxnb + 1] work-queue counters
+    uint32_t* d_skip = nullptr;                 // [nb] block left to its fix-up launch this step
+    uint32_t* d_skip_next = nullptr;            // [nb]
     uint32_t* d_calls = nullptr;
     uint32_t* h_calls_pinned = nullptr;
     hipEvent_t calls_copied = nullptr;   // the last upload of h_calls_pinned has been read
@@ -311,6 +313,8 @@ HfStepArgs<R> step_args(hf_ctx* c, int item0, int nitems, const void* A, const v
     a.queue = c->d_queue;
     a.imax_fix = c->d_imax_fix;
     a.fixed = c->d_fixed;
+    a.skip = c->d_skip;
+    a.skip_next = c->d_skip_next;
     a.imax_old = c->imax(c->cur);
     a.imax_new = c->imax(c->cur ^ 1);
     a.gmax = c->gmax(c->cur);
@@ -453,6 +457,7 @@ int ensure_partials(hf_ctx* c)
     HF_HIP(hipMemsetAsync(c->gmax(c->cur), 0, sizeof(unsigned long long) * 2 * c->nb, c->stream));
     HF_HIP(hipMemsetAsync(c->d_imax_fix, 0, sizeof(unsigned long long) * 2 * c->nb, c->stream));
     HF_HIP(hipMemsetAsync(c->d_fixed, 0, sizeof(uint32_t) * c->nb, c->stream));
+    HF_HIP(hipMemsetAsync(c->d_skip_next, 0, sizeof(uint32_t) * c->nb, c->stream));
     for (int b = 0; b < c->nb; b++) {
         int rc = launch_scan(c, b, c->buf[c->cur], c->buf[c->cur], 1, c->imax(c->cur) + 2 * b);
         if (rc) return rc;
@@ -483,9 +488,11 @@ int enqueue_steps(hf_ctx* c, int nsteps, bool timed)
         bk.imax_prev = c->imax(k);
         bk.imax_fix = c->d_imax_fix;
         bk.fixed = c->d_fixed;
+        bk.skip = c->d_skip;
+        bk.skip_next = c->d_skip_next;
         int rc = launch_ghost(c, c->d_fill, c->nfill, A, B, c->gmax(k), bk);
         if (rc) return rc;
-        rc = launch_update(c, 0, c->main_items, A, B, B, false, false, 0.0, 0.0, timed, 0, 0);
+        rc = launch_update(c, c->main_begin, c->main_items, A, B, B, false, false, 0.0, 0.0, timed, 0, 0);
         if (rc) return rc;
         for (size_t r = 0; r < c->fix_rounds.size(); r++) {
             rc = launch_update(c, c->fix_rounds[r].first, c->fix_rounds[r].second, A, B, B, false, false, 0.0, 0.0,
@@ -608,27 +615,41 @@ int hf_create(const hf_case_desc* desc, hf_ctx** out)
     // for compat MUSCL problems of several blocks, the lb-th blocks of all
     // problems for each fix-up launch lb = 1, 2, ..
     const int W = c->g == 2 ? 60 : 62;
-    const int rows = desc->tile_rows > 0 ? desc->tile_rows : 16;
+    const int rows = desc->tile_rows > 0 ? desc->tile_rows : (c->scheme == HF_MUSCL_HANCOCK ? 48 : 32);
     for (auto& P : c->probs) c->maxnb = std::max(c->maxnb, P.nb);
     c->item_begin.assign(c->nb, 0);
     c->item_count.assign(c->nb, 0);
-    auto add_block_items = [&](int b) {
+    // Items are taken from a queue in list order, so the list ends with small
+    // items: the last round of a launch then finishes together instead of
+    // waiting on a few long items.  Main launch: `rows`-row items over the
+    // first 3/4 of each block's rows, then 8-row items.  A fix-up launch
+    // computes a single block on the whole GPU: 8-row items, then 4-row items.
+    auto add_rows = [&](int b, int y0, int y1, int r) {
         const HfBlockDev& B = c->blocks[b];
         const int strips = (B.nx + W - 1) / W;
-        for (int y = 0; y < B.ny; y += rows)
-            for (int s = 0; s < strips; s++) c->items.push_back(HfItem{b, s * W, c->g + y, std::min(rows, B.ny - y)});
+        for (int y = y0; y < y1; y += r)
+            for (int s = 0; s < strips; s++) c->items.push_back(HfItem{b, s * W, c->g + y, std::min(r, y1 - y)});
     };
+    auto split = [&](int ny, int r) { return std::max(0, std::min(ny, (ny * 3 / 4) / r * r)); };
+    const int tail_rows = std::max(1, std::min(rows, 8));
     for (int b = 0; b < c->nb; b++) {
         c->item_begin[b] = (int)c->items.size();
-        add_block_items(b);
+        add_rows(b, 0, c->blocks[b].ny, rows);          // contiguous per block (class-level launches)
         c->item_count[b] = (int)c->items.size() - c->item_begin[b];
     }
-    c->main_items = (int)c->items.size();
+    const int perblock_end = (int)c->items.size();
+    for (int b = 0; b < c->nb; b++) add_rows(b, 0, split(c->blocks[b].ny, rows), rows);
+    for (int b = 0; b < c->nb; b++) add_rows(b, split(c->blocks[b].ny, rows), c->blocks[b].ny, tail_rows);
+    c->main_begin = perblock_end;
+    c->main_items = (int)c->items.size() - perblock_end;
     if (c->scheme == HF_MUSCL_HANCOCK && c->dt_mode == HF_DT_COMPAT)
         for (int lb = 1; lb < c->maxnb; lb++) {
             const int first = (int)c->items.size();
+            const int fr = std::max(1, std::min(rows, 8)), fr2 = std::max(1, std::min(rows, 4));
             for (const auto& P : c->probs)
-                if (P.nb > lb) add_block_items(P.first + lb);
+                if (P.nb > lb) add_rows(P.first + lb, 0, split(c->blocks[P.first + lb].ny, fr), fr);
+            for (const auto& P : c->probs)
+                if (P.nb > lb) add_rows(P.first + lb, split(c->blocks[P.first + lb].ny, fr), c->blocks[P.first + lb].ny, fr2);
             c->fix_rounds.emplace_back(first, (int)c->items.size() - first);
         }
 
@@ -688,6 +709,10 @@ int hf_create(const hf_case_desc* desc, hf_ctx** out)
     HF_HIPC(hipMemset(C->d_imax_fix, 0, sizeof(unsigned long long) * 2 * C->nb));
     HF_HIPC(hipMalloc(&C->d_fixed, sizeof(uint32_t) * C->nb));
     HF_HIPC(hipMemset(C->d_fixed, 0, sizeof(uint32_t) * C->nb));
+    for (uint32_t** p : {&C->d_skip, &C->d_skip_next}) {
+        HF_HIPC(hipMalloc(p, sizeof(uint32_t) * C->nb));
+        HF_HIPC(hipMemset(*p, 0, sizeof(uint32_t) * C->nb));
+    }
     HF_HIPC(hipMalloc(&C->d_queue, sizeof(uint32_t) * (C->maxnb + 1)));
     HF_HIPC(hipMemset(C->d_queue, 0, sizeof(uint32_t) * (C->maxnb + 1)));
     {
@@ -722,7 +747,7 @@ void hf_destroy(hf_ctx* c)
     for (void* p : {c->buf[0], c->buf[1], (void*)c->d_blocks, (void*)c->d_probs, (void*)c->d_items, (void*)c->d_fill,
                     (void*)c->d_copy, c->d_consts, (void*)c->d_imax, (void*)c->d_gmax, (void*)c->d_scan,
                     (void*)c->d_calls, (void*)c->d_hist, (void*)c->d_part, c->d_resid, (void*)c->d_imax_fix,
-                    (void*)c->d_fixed, (void*)c->d_queue})
+                    (void*)c->d_fixed, (void*)c->d_queue, (void*)c->d_skip, (void*)c->d_skip_next})
         if (p) hipFree(p);
     if (c->h_calls_pinned) hipHostFree(c->h_calls_pinned);
     if (c->calls_copied) hipEventDestroy(c->calls_copied);

@@ -65,6 +65,8 @@ struct HfStepArgs {
     uint32_t* queue;                     // work-queue counter of this launch (zero at launch)
     unsigned long long* imax_fix;        // [nblocks][2] maxima of blocks recomputed by a fix-up launch
     uint32_t* fixed;                     // [nblocks] block was recomputed by a fix-up launch this step
+    uint32_t* skip;                      // [nblocks] main launch leaves the block to its fix-up launch
+    uint32_t* skip_next;                 // [nblocks] the fix-up launch's verdict for the next step
 };
 
 // Step bookkeeping done by the first workgroup of a ghost-fill launch.
@@ -83,4 +85,6 @@ struct GhostBook {
     unsigned long long* imax_prev;   // maxima of the state this step starts from
     unsigned long long* imax_fix;
     uint32_t* fixed;
+    uint32_t* skip;               // skip <- skip_next for the step that starts
+    uint32_t* skip_next;
 };

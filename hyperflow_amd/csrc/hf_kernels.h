@@ -50,15 +50,24 @@ __device__ __forceinline__ double shfl_dn1(double x)
 __device__ __forceinline__ float shfl_up1(float x) { return __int_as_float(dpp_up(__float_as_int(x))); }
 __device__ __forceinline__ float shfl_dn1(float x) { return __int_as_float(dpp_dn(__float_as_int(x))); }
 
-// Next work item of a persistent launch.  Every lane executes the atomic
-// (only lane 0 adds 1) so that no lane-divergent region surrounds the loop's
-// exit test; lane 0's old value is the wave's ticket.  A lane-0-only branch
-// here let the compiler's control-flow structurizer keep lanes 1..63 in the
-// loop after lane 0 left it.
+// Work items of a persistent launch: wave w first takes item w (no atomic,
+// so a launch does not open with every wave hitting one counter), then
+// tickets from the queue counter, offset by the number of waves.  Every lane
+// executes the atomic (only lane 0 adds 1) so that no lane-divergent region
+// surrounds the loop's exit test; lane 0's old value is the wave's ticket.  A
+// lane-0-only branch here let the compiler's control-flow structurizer keep
+// lanes 1..63 in the loop after lane 0 left it.
+__device__ __forceinline__ unsigned total_waves() { return gridDim.x * (blockDim.x >> 6); }
+__device__ __forceinline__ unsigned first_ticket()
+{
+    // wave-uniform, but derived from threadIdx: say so, or the whole item loop
+    // is compiled as divergent (vector loads of the item and block records)
+    return __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+}
 __device__ __forceinline__ unsigned next_ticket(uint32_t* queue, int lane)
 {
     const unsigned t = atomicAdd(queue, lane == 0 ? 1u : 0u);
-    return __builtin_amdgcn_readfirstlane(t);
+    return __builtin_amdgcn_readfirstlane(t) + total_waves();
 }
 
 // Per-wave maxima into the block's slots; every lane holds the reduced value,
@@ -180,11 +189,24 @@ __device__ __forceinline__ void muscl_item(const HfStepArgs<typename M::R>& a, c
 }
 
 // Persistent MUSCL-Hancock launch: waves take work items from a queue until
-// it is empty.  The main launch of a step uses the speculative predictor dt
-// (every block at U^n); a fix-up launch (fix_lb > 0) recomputes the fix_lb-th
-// block of a problem only when the reference's dt for it -- earlier blocks
-// already at U^{n+1} -- differs, so results equal the reference's sequential
-// sweep bit for bit.
+// it is empty.  The main launch of a step computes every block with the
+// speculative predictor dt (every block at U^n), except blocks marked skip[b]
+// (their speculation failed in the previous step).  A fix-up launch
+// (fix_lb > 0) then computes the fix_lb-th block of each problem with the
+// reference's dt -- earlier blocks at their final U^{n+1} -- when the block
+// was skipped or its speculative dt differs, so results equal the
+// reference's sequential sweep bit for bit.  The fix-up launch also records
+// whether the block will need it again (skip_next).
+template <class R>
+__device__ __forceinline__ bool needs_fix(const HfStepArgs<R>& a, int b, const HfBlockDev& bd, double& exact)
+{
+    double s0, spec;
+    long long k0, k1;
+    step_dts<R>(a, b, bd, true, DT_SPEC, s0, spec, k0, k1);
+    step_dts<R>(a, b, bd, true, DT_FINAL, s0, exact, k0, k1);
+    return a.skip[b] || spec != exact;
+}
+
 template <class M, int LIM, bool RESID>
 __global__ __launch_bounds__(256) void k_muscl(const HfStepArgs<typename M::R> a)
 {
@@ -192,23 +214,27 @@ __global__ __launch_bounds__(256) void k_muscl(const HfStepArgs<typename M::R> a
     const int lane = threadIdx.x & 63;
     const bool fix = a.fix_lb > 0;
     if (fix) {
-        // leave at once unless some problem's fix_lb-th block needs recomputing
+        // leave at once unless some problem's fix_lb-th block needs computing
         bool any = false;
         for (int p = 0; p < a.nprob; p++) {
             const HfProbDev pr = a.probs[p];
             if (pr.nb <= a.fix_lb) continue;
             const int b = pr.first + a.fix_lb;
             const HfBlockDev bd = a.blocks[b];
-            double s0, sp, t0, tp;
+            double s0, spec, exact;
             long long k0, k1;
-            step_dts<R>(a, b, bd, true, false, s0, sp, k0, k1);
-            step_dts<R>(a, b, bd, true, true, t0, tp, k0, k1);
-            any = any || (sp != tp);
+            step_dts<R>(a, b, bd, true, DT_SPEC, s0, spec, k0, k1);
+            step_dts<R>(a, b, bd, true, DT_FINAL, s0, exact, k0, k1);
+            a.skip_next[b] = spec != exact ? 1u : 0u;    // same value from every wave
+            any = any || a.skip[b] || spec != exact;
         }
         if (!any) return;
     }
-    for (;;) {
-        const unsigned idx = next_ticket(a.queue, lane);
+    int cached = -1;          // block whose dts (and fix verdict) this wave has computed
+    double c_step = 0.0, c_pred = 0.0;
+    long long c_kstep = 0, c_kpred = 0;
+    bool c_need = false;
+    for (unsigned idx = first_ticket();; idx = next_ticket(a.queue, lane)) {
         if (idx >= (unsigned)a.nitems) break;
         const HfItem it = a.items[a.item0 + idx];
         double dt_step, dt_pred;
@@ -216,14 +242,21 @@ __global__ __launch_bounds__(256) void k_muscl(const HfStepArgs<typename M::R> a
             dt_step = a.fixed_step;
             dt_pred = a.fixed_pred;
         } else {
+            if (!fix && a.skip[it.block]) continue;   // computed exactly by its fix-up launch
             const HfBlockDev bd = a.blocks[it.block];
-            long long kstep, kpred;
-            step_dts<R>(a, it.block, bd, true, fix, dt_step, dt_pred, kstep, kpred);
+            if (it.block != cached) {
+                cached = it.block;
+                step_dts<R>(a, it.block, bd, true, fix ? DT_FINAL : DT_SPEC, c_step, c_pred, c_kstep, c_kpred);
+                if (fix) {
+                    double exact;
+                    c_need = needs_fix<R>(a, it.block, bd, exact);
+                }
+            }
+            dt_step = c_step;
+            dt_pred = c_pred;
+            const long long kstep = c_kstep, kpred = c_kpred;
             if (fix) {
-                double s0, sp;
-                long long k0, k1;
-                step_dts<R>(a, it.block, bd, true, false, s0, sp, k0, k1);
-                if (sp == dt_pred) continue;          // main launch already used this dt
+                if (!c_need) continue;
                 a.fixed[it.block] = 1u;
             }
             if (bd.problem == 0 && it.x0 == 0 && it.y0 == bd.g) {   // same value from every lane
@@ -317,8 +350,7 @@ __global__ __launch_bounds__(256) void k_godunov(const HfStepArgs<typename M::R>
 {
     typedef typename M::R R;
     const int lane = threadIdx.x & 63;
-    for (;;) {
-        const unsigned idx = next_ticket(a.queue, lane);
+    for (unsigned idx = first_ticket();; idx = next_ticket(a.queue, lane)) {
         if (idx >= (unsigned)a.nitems) break;
         const HfItem it = a.items[a.item0 + idx];
         double dt_step, dt_pred;
@@ -327,7 +359,7 @@ __global__ __launch_bounds__(256) void k_godunov(const HfStepArgs<typename M::R>
         } else {
             const HfBlockDev bd = a.blocks[it.block];
             long long kstep, kpred;
-            step_dts<R>(a, it.block, bd, false, false, dt_step, dt_pred, kstep, kpred);
+            step_dts<R>(a, it.block, bd, false, DT_SPEC, dt_step, dt_pred, kstep, kpred);
             if (bd.problem == 0 && bd.lb == 0 && it.x0 == 0 && it.y0 == bd.g && kstep >= 1 &&
                 kstep <= a.hist_cap)
                 a.hist[kstep - 1] = dt_step;
@@ -366,6 +398,7 @@ __global__ __launch_bounds__(256) void k_ghost(const HfGhost* __restrict__ tab, 
                 book.imax_prev[2 * b] = book.imax_fix[2 * b];
                 book.imax_prev[2 * b + 1] = book.imax_fix[2 * b + 1];
             }
+            book.skip[b] = book.skip_next[b];
             book.fixed[b] = 0u;
             book.imax_fix[2 * b] = 0ull;
             book.imax_fix[2 * b + 1] = 0ull;

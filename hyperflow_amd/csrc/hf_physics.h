@@ -275,21 +275,27 @@ __device__ __forceinline__ double cfl_eff(double cfl, long long call)
 
 // Largest wave speeds of the new interior of block c after the step's
 // fix-up launches so far.
+// New maxima of block c as seen by a predictor dt: none (every block at U^n,
+// the speculative value) or final (after the main launch and the fix-ups so
+// far).
+enum { DT_SPEC = 0, DT_FINAL = 2 };
+
 template <class R>
-__device__ __forceinline__ double new_max(const HfStepArgs<R>& a, int c, int axis)
+__device__ __forceinline__ double new_max(const HfStepArgs<R>& a, int c, int axis, int view)
 {
-    return bitsd(a.fixed[c] ? a.imax_fix[2 * c + axis] : a.imax_new[2 * c + axis]);
+    // written by earlier launches only
+    return bitsd((view == DT_FINAL && a.fixed[c]) ? a.imax_fix[2 * c + axis] : a.imax_new[2 * c + axis]);
 }
 
-// Step dt and predictor dt of a block.  The predictor dt follows the
-// reference's compat semantics when final_new is set: blocks of the problem
-// before this one are already at U^{n+1} (simulation.cpp:106-110 with
-// muscl_hancock.cpp:246).  Without final_new every block is taken at U^n,
-// which is the speculative value the main launch uses; a fix-up launch
-// recomputes a block whenever the two differ.
+// Step dt and predictor dt of a block.  With view DT_FINAL the predictor dt
+// follows the reference's compat semantics: blocks of the
+// problem before this one are already at U^{n+1} (simulation.cpp:106-110 with
+// muscl_hancock.cpp:246).  With DT_SPEC every block is taken at U^n, the
+// speculative value of the main launch; a fix-up launch recomputes a block
+// whenever the dt it used differs from the DT_FINAL one.
 template <class R>
 __device__ __forceinline__ void step_dts(const HfStepArgs<R>& a, int block, const HfBlockDev& bd, bool muscl,
-                                         bool final_new, double& dt_step, double& dt_pred, long long& kstep,
+                                         int view, double& dt_step, double& dt_pred, long long& kstep,
                                          long long& kpred)
 {
     const HfProbDev pr = a.probs[bd.problem];
@@ -305,9 +311,9 @@ __device__ __forceinline__ void step_dts(const HfStepArgs<R>& a, int block, cons
         dtx = smin(dtx, sx);
         dty = smin(dty, sy);
         if (compat) {
-            if (final_new && c < block) {
-                ptx = smin(ptx, cdx / smax(new_max(a, c, 0), gx));
-                pty = smin(pty, cdy / smax(new_max(a, c, 1), gy));
+            if (view != DT_SPEC && c < block) {
+                ptx = smin(ptx, cdx / smax(new_max(a, c, 0, view), gx));
+                pty = smin(pty, cdy / smax(new_max(a, c, 1, view), gy));
             } else {
                 ptx = smin(ptx, sx);
                 pty = smin(pty, sy);
