@@ -7,6 +7,13 @@ synthetic initial data (the inflow state), one MI355X.
 A step is one full time step of the path: ghost fill, CFL timestep reduction
 and the eight fused per-block MUSCL-HLLC updates.  The timed region starts
 with the state resident in HBM.  Prints one JSON line.
+
+Other workloads of the same path (not the headline line):
+    --workload c1   Godunov + HLLC, 400x400 FP64, transmissive, CFL 0.8
+    --workload c3   MUSCL-Hancock + Van Leer, 8192x8192 (--precision fp32|fp64)
+    --workload c4   512 independent problems of U[64,512]^2 cells, MUSCL FP32,
+                    one persistent launch per step, HIP graph; also times
+                    looping the same problems one context at a time
 """
 from __future__ import annotations
 
@@ -65,14 +72,64 @@ def traffic_from_profiles(workload: str):
         return None
 
 
+def build_workload(name: str, n: int, precision: str, seed: int = 0):
+    from hyperflow_amd import case as C
+    if name == "c1":
+        case = C.config1(seed)
+        return [case], [case.initial_fields()[0]], "Godunov+HLLC FP64, 400x400, transmissive, CFL 0.8"
+    if name == "c2":
+        case = C.config2(n)
+        return [case], case.initial_fields(), (
+            f"MUSCL-Hancock+MinBee+HLLC {precision.upper()}, 8 blocks of {n}x{n} step channel (Dirichlet inflow, "
+            "reflective walls, transmissive outflow), CFL 0.5, reference-compatible dt")
+    if name == "c3":
+        n3 = 8192 if n == 1024 else n
+        case = C.config3(seed, n3)
+        return [case], C.perturbed_fields(case, seed), (
+            f"MUSCL-Hancock+VanLeer+HLLC {precision.upper()}, single block {n3}x{n3}, seeded quadrants with "
+            "sinusoidal density perturbation, CFL 0.5")
+    if name == "c4":
+        cases = [C.quad_case(nx, ny, 1000 + k, scheme="MUSCLHancock", limiter="VanLeer", cfl=0.5)
+                 for k, (nx, ny) in enumerate(C.batched_sizes(512, seed))]
+        return cases, [c.initial_fields()[0] for c in cases], (
+            f"512 independent seeded 2D Riemann problems, sides U[64,512], MUSCL-HLLC {precision.upper()}, "
+            "one persistent tile-queue launch per step, HIP-graph captured")
+    raise SystemExit(f"unknown workload {name}")
+
+
+def looped_baseline(cases, fields, precision, steps):
+    """Each problem in its own context, advanced one after another."""
+    from hyperflow_amd import native
+    solvers = []
+    for c, f in zip(cases, fields):
+        s = native.Solver(c, precision=precision)
+        s.upload(0, f)
+        solvers.append(s)
+    for s in solvers:
+        s.run(2)
+    for s in solvers:
+        s.synchronize()
+    t = time.perf_counter()
+    for s in solvers:
+        s.run(steps)
+    for s in solvers:
+        s.synchronize()
+    el = time.perf_counter() - t
+    for s in solvers:
+        s.close()
+    return el
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--n", type=int, default=1024, help="cells per block side")
-    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
-    ap.add_argument("--tile-rows", type=int, default=0)
+    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4"], default="c2")
+    ap.add_argument("--n", type=int, default=1024, help="cells per block side (c2), grid side (c3)")
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default=None)
+    ap.add_argument("--tile-rows", type=int, default=None, help="default: autotuned value if cached")
+    ap.add_argument("--graph", action="store_true", help="capture the timed steps in a HIP graph")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
 
@@ -82,33 +139,43 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    from hyperflow_amd import case as C
-    from hyperflow_amd import native
+    from hyperflow_amd import autotune, native
 
-    case = C.config2(args.n)
-    solver = native.Solver(case, precision=args.precision, tile_rows=args.tile_rows)
-    solver.upload_all(case.initial_fields())
-    solver.run(args.warmup)
+    precision = args.precision or ("fp32" if args.workload == "c4" else "fp64")
+    cases, fields, workload_text = build_workload(args.workload, args.n, precision)
+    tile_rows = args.tile_rows if args.tile_rows is not None else autotune.cached_rows(cases, precision)
+    graph = args.graph or args.workload == "c4"
+
+    solver = native.Solver(cases, precision=precision, tile_rows=tile_rows)
+    solver.upload_all(fields)
+    solver.run(args.warmup, graph=graph)
     solver.synchronize()
 
-    solver.profile(True)
+    solver.profile(not graph)
     solver.kernel_stats()                         # reset counters
     t0 = time.perf_counter()
-    solver.run(args.steps)
+    solver.run(args.steps, graph=graph)
     solver.synchronize()
     t1 = time.perf_counter()
     launches, kernel_ms = solver.kernel_stats()
     solver.profile(False)
 
     wall = t1 - t0
-    cells = case.interior_cells
+    cells = sum(c.interior_cells for c in cases)
     value = cells * args.steps / wall / 1e6
-    avg_ms = kernel_ms / max(launches, 1)
-    cells_per_launch = cells * args.steps / max(launches, 1)
-    alg_bytes = cells_per_launch * BYTES_PER_CELL[args.precision]
+    bpc = BYTES_PER_CELL[precision]
+    if launches:
+        avg_ms = kernel_ms / launches
+        cells_per_launch = cells * args.steps / launches
+    else:                                         # graph mode: no per-launch events
+        avg_ms = wall * 1e3 / args.steps
+        cells_per_launch = cells
+    alg_bytes = cells_per_launch * bpc
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
-    workload = f"step_channel_{args.n}_{args.precision}"
-    traffic = traffic_from_profiles(workload)
+    wl = f"{args.workload}_{args.n}_{precision}"
+    traffic = traffic_from_profiles(wl)
+    kernel = ("k_muscl (all update launches of a step: main + fix-ups; bytes = the step's cells / launches)"
+              if cases[0].scheme == "MUSCLHancock" else "k_godunov")
 
     line = {
         "metric": METRIC,
@@ -121,19 +188,24 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32",
-        "data": "synthetic: uniform Mach 3 inflow state in a channel with a step",
-        "config": {"workload": f"MUSCL-Hancock+MinBee+HLLC {args.precision.upper()}, 8 blocks of "
-                               f"{args.n}x{args.n} step channel (Dirichlet inflow, reflective walls, "
-                               f"transmissive outflow), CFL 0.5, reference-compatible dt",
-                   "blocks": 8, "block_cells": [args.n, args.n], "scheme": "MUSCLHancock", "limiter": "MinBee",
-                   "riemann": "HLLC", "cfl": 0.5, "dt_mode": "compat"},
+        "dtype": "f64" if precision == "fp64" else "f32",
+        "data": "synthetic (seeded initial states of the named workload)",
+        "config": {"workload": workload_text, "blocks": sum(len(c.blocks) for c in cases),
+                   "interior_cells": cells, "scheme": cases[0].scheme, "limiter": cases[0].limiter,
+                   "riemann": "HLLC", "cfl": cases[0].cfl, "tile_rows": tile_rows or "default",
+                   "hip_graph": graph},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_muscl (per-block update)", "avg_launch_ms": round(avg_ms, 5),
-                     "launches": launches, "algorithmic_bytes_per_launch": int(alg_bytes)},
+                     "kernel": kernel if launches else "whole step (graph)", "avg_launch_ms": round(avg_ms, 5),
+                     "launches": launches, "algorithmic_bytes_per_launch": int(alg_bytes),
+                     "bytes_per_cell": bpc},
     }
-    if not args.no_cpu_baseline:
+    if args.workload == "c4":
+        solver.close()
+        loop = looped_baseline(cases, fields, precision, args.steps)
+        line["looped_per_problem_s"] = round(loop, 4)
+        line["batched_speedup_vs_looped"] = round(loop / wall, 2)
+    if not args.no_cpu_baseline and args.workload == "c2":
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)
 

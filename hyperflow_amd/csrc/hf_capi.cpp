@@ -496,7 +496,7 @@ int enqueue_steps(hf_ctx* c, int nsteps, bool timed)
         if (rc) return rc;
         for (size_t r = 0; r < c->fix_rounds.size(); r++) {
             rc = launch_update(c, c->fix_rounds[r].first, c->fix_rounds[r].second, A, B, B, false, false, 0.0, 0.0,
-                               false, (int)r + 1, (int)r + 1);
+                               timed, (int)r + 1, (int)r + 1);
             if (rc) return rc;
         }
         c->cur ^= 1;

@@ -136,14 +136,17 @@ __device__ __forceinline__ void muscl_item(const HfStepArgs<typename M::R>& a, c
         const State<R> sr = make_state<M>(lo2, gam, g1);
         hllc<M, false>(sl, sr, Gs);
     }
-    // window: uc = U(y), up = U(y+1)
+    // window: uc = U(y), up = U(y+1), un = U(y+2) (loaded one iteration ahead
+    // so that HBM latency overlaps the row's arithmetic)
     for (int k = 0; k < 4; k++) { uc[k] = up[k]; up[k] = un[k]; }
+    ROW(y0 + 2, un);
 
     R* Ob = RESID ? a.O : a.O + bd.off;
     double mx = 0.0, my = 0.0;
     for (int r = 0; r < it.rows; r++) {
         const int y = y0 + r;
-        ROW(y + 2, un);
+        R nn[4];
+        ROW(y + 3 < ty ? y + 3 : ty - 1, nn);     // next iteration's U(y+2)
 
         // y direction: reconstruct cell y+1, flux through face y+1/2
         R Bn[4], Tn[4], Gn[4];
@@ -182,7 +185,7 @@ __device__ __forceinline__ void muscl_item(const HfStepArgs<typename M::R>& a, c
                 my = dmax_nan(my, dy_);
             }
         }
-        for (int k = 0; k < 4; k++) { Gs[k] = Gn[k]; Tc[k] = Tn[k]; uc[k] = up[k]; up[k] = un[k]; }
+        for (int k = 0; k < 4; k++) { Gs[k] = Gn[k]; Tc[k] = Tn[k]; uc[k] = up[k]; up[k] = un[k]; un[k] = nn[k]; }
     }
     if (!RESID) store_maxima(imax_out, it.block, wave_max(mx), wave_max(my));
 #undef ROW
@@ -304,12 +307,14 @@ __device__ __forceinline__ void godunov_item(const HfStepArgs<typename M::R>& a,
     R Gs[4];
     hllc<M, false>(sb, sc, Gs);                    // face y0-1/2
 
+    R un[4];
+    ROW(y0 + 1, un);                               // loaded one iteration ahead
     R* Ob = RESID ? a.O : a.O + bd.off;
     double mx = 0.0, my = 0.0;
     for (int r = 0; r < it.rows; r++) {
         const int y = y0 + r;
-        R un[4];
-        ROW(y + 1, un);
+        R nn[4];
+        ROW(y + 2 < ty ? y + 2 : ty - 1, nn);     // next iteration's U(y+1)
         const State<R> sn = make_state<M>(un, gam, g1);
         R Gn[4], Fw[4], Fe[4];
         hllc<M, false>(sc, sn, Gn);                // face y+1/2
@@ -337,7 +342,7 @@ __device__ __forceinline__ void godunov_item(const HfStepArgs<typename M::R>& a,
                 my = dmax_nan(my, dy_);
             }
         }
-        for (int k = 0; k < 4; k++) Gs[k] = Gn[k];
+        for (int k = 0; k < 4; k++) { Gs[k] = Gn[k]; un[k] = nn[k]; }
         sc = sn;
     }
     if (!RESID) store_maxima(imax_out, it.block, wave_max(mx), wave_max(my));
