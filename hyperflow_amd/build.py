@@ -22,20 +22,24 @@ LIB = os.path.join(HERE, "libhyperflow_amd.so")
 ARCH = os.environ.get("HF_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
-COMMON = ["-std=c++17", "-fPIC", "-fvisibility=hidden", f"-I{os.path.join(ROOT, 'include')}", f"-I{CSRC}"]
+COMMON = ["-std=c++17", "-fPIC", f"-I{os.path.join(ROOT, 'include')}", f"-I{CSRC}"]
 DEVICE = [f"--offload-arch={ARCH}", "-O3", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
+# (source, flags, hidden visibility): the C++ drop-in classes are exported
 UNITS = [
-    ("hf_kernels_f64.hip", DEVICE + ["-ffp-contract=off"]),
-    ("hf_kernels_f32.hip", DEVICE + ["-ffp-contract=off"]),
-    ("hf_capi.cpp", ["-O2", "-ffp-contract=off"]),
+    ("hf_kernels_f64.hip", DEVICE + ["-ffp-contract=off"], True),
+    ("hf_kernels_f32.hip", DEVICE + ["-ffp-contract=off"], True),
+    ("hf_capi.cpp", ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-O2", "-ffp-contract=off"], True),
+    ("host/hf_json.cpp", ["-x", "c++", "-O2"], False),
+    ("host/hf_host.cpp", ["-x", "c++", "-O2", "-ffp-contract=off"], False),
 ]
+CLI = os.path.join(HERE, "bin", "hyperflow")
 
 
-def _compile(unit, flags, outdir):
+def _compile(unit, flags, hidden, outdir):
     src = os.path.join(CSRC, unit)
-    obj = os.path.join(outdir, unit + ".o")
-    cmd = [HIPCC] + COMMON + flags + ["-c", src, "-o", obj]
+    obj = os.path.join(outdir, os.path.basename(unit) + ".o")
+    cmd = [HIPCC] + COMMON + (["-fvisibility=hidden"] if hidden else []) + flags + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     msgs = [ln for ln in r.stdout.splitlines() if "packed-fp32-ops" not in ln]
     if r.returncode != 0:
@@ -46,16 +50,24 @@ def _compile(unit, flags, outdir):
 def build(verbose: bool = False) -> str:
     outdir = os.path.join(HERE, "_build")
     os.makedirs(outdir, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=len(UNITS)) as ex:
-        objs = list(ex.map(lambda u: _compile(u[0], u[1], outdir), UNITS))
+    with ThreadPoolExecutor(max_workers=min(8, len(UNITS))) as ex:
+        objs = list(ex.map(lambda u: _compile(u[0], u[1], u[2], outdir), UNITS))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout)
     os.replace(tmp, LIB)
+    # the `hyperflow` command-line driver, linked against the library
+    os.makedirs(os.path.dirname(CLI), exist_ok=True)
+    cli_obj = _compile("host/hf_cli.cpp", ["-x", "c++", "-O2"], False, outdir)
+    cmd = [HIPCC, "-o", CLI + ".tmp", cli_obj, f"-L{HERE}", "-lhyperflow_amd", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("CLI link failed:\n" + r.stdout)
+    os.replace(CLI + ".tmp", CLI)
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {CLI}")
     return LIB
 
 

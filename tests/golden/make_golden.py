@@ -88,3 +88,34 @@ def main():
 
 if __name__ == "__main__":
     main()
+
+
+def cli_goldens():
+    """Per-frame SHA-256 of the results_%06d.csv files the reference's own
+    program (oracle/_ref/ref_cli) writes for shipped cases."""
+    import hashlib
+    import subprocess
+    import tempfile
+    ref_cli = os.path.join(ROOT, "oracle", "_ref", "ref_cli")
+    out = {}
+    for p in ("shock_tubes/toro/toro_shock_tube_01_godunov_hllc_100x5.json",
+              "shock_tubes/toro/toro_shock_tube_01_muscl_hancock_vanleer_hllc_100x5.json"):
+        src = os.path.join("/root/reference/cases/fvm_2d_cart", p)
+        with tempfile.TemporaryDirectory() as td:
+            case = C.Case.load(src)
+            cj = os.path.join(td, "case.json")
+            case.write_json(cj)
+            r = subprocess.run([ref_cli, cj], cwd=td, stdout=subprocess.PIPE, text=True, check=True)
+            steps = sum(1 for ln in r.stdout.splitlines() if ln.startswith("Steps:"))
+            frames = sorted(f for f in os.listdir(td) if f.startswith("results_"))
+            hashes = {f: hashlib.sha256(open(os.path.join(td, f), "rb").read()).hexdigest() for f in frames}
+            first = open(os.path.join(td, frames[1])).read()
+        out[os.path.basename(p)[:-5]] = {"case": case.to_json(), "progress_lines": steps, "frames": hashes,
+                                         "frame_000001": first}
+    with open(os.path.join(HERE, "cli_frames.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+    print("wrote cli_frames.json")
+
+
+if __name__ == "__main__" and "--cli" in sys.argv:
+    cli_goldens()
