@@ -14,6 +14,8 @@ Other workloads of the same path (not the headline line):
     --workload c4   512 independent problems of U[64,512]^2 cells, MUSCL FP32,
                     one persistent launch per step, HIP graph; also times
                     looping the same problems one context at a time
+    --workload copy a 1 GiB device copy: measured copy bandwidth, and the
+                    calibration run for the HBM traffic counters
 """
 from __future__ import annotations
 
@@ -60,9 +62,32 @@ def cpu_baseline(sample_n: int = 256, sample_steps: int = 5):
             "sample": sample + " (CPU oracle, OpenMP)"}
 
 
+def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 20):
+    """A plain device copy of a known size (read nbytes + write nbytes per
+    launch), timed with HIP events: the measured HBM copy bandwidth, and the
+    calibration kernel for the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    a = torch.full((nbytes // 4,), 1.0, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(reps):
+        b.copy_(a)
+    stop.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / reps
+    return {"metric": "device copy bandwidth", "value": round(2 * nbytes / (ms * 1e-3) / 1e9, 1), "unit": "GB/s",
+            "bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes, "avg_launch_ms": round(ms, 4),
+            "launches": reps + 3}
+
+
 def traffic_from_profiles(workload: str):
     """HBM bytes per update launch measured with rocprofv3 --pmc (FETCH_SIZE and
-    WRITE_SIZE, calibrated against a copy kernel), committed under profiles/."""
+    WRITE_SIZE in separate runs, each scaled by what the counter reads for a
+    copy of known size), committed in profiles/hbm_traffic.json."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     try:
         with open(path) as f:
@@ -125,7 +150,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4"], default="c2")
+    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "copy"], default="c2")
     ap.add_argument("--n", type=int, default=1024, help="cells per block side (c2), grid side (c3)")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default=None)
     ap.add_argument("--tile-rows", type=int, default=None, help="default: autotuned value if cached")
@@ -138,6 +163,9 @@ def main():
               "(the path would shard by blocks with a halo exchange and a dt min-allreduce, not measured here)",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "copy":
+        print(json.dumps(copy_bandwidth()), flush=True)
+        return
 
     from hyperflow_amd import autotune, native
 
@@ -172,8 +200,8 @@ def main():
         cells_per_launch = cells
     alg_bytes = cells_per_launch * bpc
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
-    wl = f"{args.workload}_{args.n}_{precision}"
-    traffic = traffic_from_profiles(wl)
+    # counters were taken at the default sizes (1024 per block side, 8192 for c3)
+    traffic = traffic_from_profiles(f"{args.workload}_{precision}") if args.n == 1024 else None
     kernel = ("k_muscl (all update launches of a step: main + fix-ups; bytes = the step's cells / launches)"
               if cases[0].scheme == "MUSCLHancock" else "k_godunov")
 
