@@ -29,7 +29,9 @@ DEVICE = [f"--offload-arch={ARCH}", "-O3", "-Xclang", "-target-feature", "-Xclan
 UNITS = [
     ("hf_kernels_f64.hip", DEVICE + ["-ffp-contract=off"], True),
     ("hf_kernels_f32.hip", DEVICE + ["-ffp-contract=off"], True),
+    ("hf_cnn.hip", DEVICE + ["-ffp-contract=off"], True),
     ("hf_capi.cpp", ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-O2", "-ffp-contract=off"], True),
+    ("hf_learned.cpp", ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-O2"], True),
     ("host/hf_json.cpp", ["-x", "c++", "-O2"], False),
     ("host/hf_host.cpp", ["-x", "c++", "-O2", "-ffp-contract=off"], False),
 ]

@@ -22,25 +22,22 @@
 #include <vector>
 
 #include "../../include/hyperflow.h"
+#include "../../include/hyperflow_learned.h"
 #include "hf_common.h"
+#include "hf_context.h"
 #include "hf_launch.h"
-
-namespace {
 
 thread_local std::string g_err;
 
-int fail(int code, const std::string& msg)
+int hf_fail(int code, const std::string& msg)
 {
     g_err = msg;
     return code;
 }
 
-#define HF_HIP(call)                                                                         \
-    do {                                                                                     \
-        hipError_t e_ = (call);                                                              \
-        if (e_ != hipSuccess)                                                                \
-            return fail(HF_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));        \
-    } while (0)
+namespace {
+
+inline int fail(int code, const std::string& msg) { return hf_fail(code, msg); }
 
 inline double smin(double a, double b) { return (b < a) ? b : a; }
 inline double smax(double a, double b) { return (a < b) ? b : a; }
@@ -62,61 +59,6 @@ struct Prov {
 
 }  // namespace
 
-struct hf_ctx {
-    int scheme = 0, limiter = 0, prec = 0, dt_mode = 0, g = 1, nb = 0, np = 0;
-    double gamma = 1.4, cfl = 0.5;
-    size_t esize = 8;
-    std::vector<hf_block_desc> bdesc;
-    std::vector<HfBlockDev> blocks;
-    std::vector<HfProbDev> probs;
-    std::vector<HfItem> items;
-    std::vector<int> item_begin, item_count;      // per block
-    std::vector<std::pair<int, int>> fix_rounds;  // item ranges of the fix-up launches (lb = 1, 2, ..)
-    int main_begin = 0, main_items = 0;           // item range of a step's main launch
-    int maxnb = 1;
-    int wg_cap = 1024;                            // resident workgroups of an update launch
-    int64_t total = 0;                            // elements per state buffer
-    void* buf[2] = {nullptr, nullptr};
-    int cur = 0;
-    bool ghosts_pending = false;   // buf[cur^1]'s ghosts equal buf[cur]'s (apply_bcs done)
-    bool partials_valid = false;   // imax[cur] describes buf[cur], gmax[cur] is zero
-
-    HfBlockDev* d_blocks = nullptr;
-    HfProbDev* d_probs = nullptr;
-    HfItem* d_items = nullptr;
-    HfGhost* d_fill = nullptr;     // ghost-fill table
-    HfGhost* d_copy = nullptr;     // ghost identity table (copy ghosts between buffers)
-    int nfill = 0, ncopy = 0;
-    void* d_consts = nullptr;
-    unsigned long long* d_imax = nullptr;   // [2][nb][2]
-    unsigned long long* d_gmax = nullptr;   // [2][nb][2]
-    unsigned long long* d_scan = nullptr;   // [nb][2] scan results + [nb][2] scratch maxima
-    unsigned long long* d_imax_fix = nullptr;   // [nb][2]
-    uint32_t* d_fixed = nullptr;                // [nb]
-    uint32_t* d_queue = nullptr;                // [maxnb + 1] work-queue counters
-    uint32_t* d_skip = nullptr;                 // [nb] block left to its fix-up launch this step
-    uint32_t* d_skip_next = nullptr;            // [nb]
-    uint32_t* d_calls = nullptr;
-    uint32_t* h_calls_pinned = nullptr;
-    hipEvent_t calls_copied = nullptr;   // the last upload of h_calls_pinned has been read
-    double* d_hist = nullptr;
-    int hist_cap = 1 << 20;
-    double* d_part = nullptr;
-    int part_wg = 256;
-    void* d_resid = nullptr;
-    int64_t resid_elems = 0;
-    std::vector<int64_t> calls;     // host mirror of the TimeStep call counters
-    hipStream_t stream = nullptr;
-    std::map<std::pair<int, int>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
-
-    bool profile = false;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending, ev_free;
-    int64_t prof_launches = 0;
-    double prof_ms = 0.0;
-
-    unsigned long long* imax(int parity) { return d_imax + (size_t)parity * nb * 2; }
-    unsigned long long* gmax(int parity) { return d_gmax + (size_t)parity * nb * 2; }
-};
 
 namespace {
 
@@ -466,8 +408,10 @@ int ensure_partials(hf_ctx* c)
     return HF_OK;
 }
 
-// Queue nsteps device-resident steps starting from buffer parity c0.
-int enqueue_steps(hf_ctx* c, int nsteps, bool timed)
+// Queue nsteps device-resident steps starting from buffer parity c0.  With
+// fixed, every step and MUSCL predictor uses dt and no TimeStep call is
+// counted (no fix-up launches are needed).
+int enqueue_steps(hf_ctx* c, int nsteps, bool timed, bool fixed = false, double dt = 0.0)
 {
     const int c0 = c->cur;
     for (int s = 0; s < nsteps; s++) {
@@ -478,7 +422,7 @@ int enqueue_steps(hf_ctx* c, int nsteps, bool timed)
         bk.calls = c->d_calls;
         bk.probs = c->d_probs;
         bk.nprob = c->np;
-        bk.bump = 1;
+        bk.bump = fixed ? 0 : 1;
         bk.clear0 = c->gmax(k ^ 1);
         bk.clear1 = c->imax(k ^ 1);
         bk.nclear = 2 * c->nb;
@@ -492,9 +436,9 @@ int enqueue_steps(hf_ctx* c, int nsteps, bool timed)
         bk.skip_next = c->d_skip_next;
         int rc = launch_ghost(c, c->d_fill, c->nfill, A, B, c->gmax(k), bk);
         if (rc) return rc;
-        rc = launch_update(c, c->main_begin, c->main_items, A, B, B, false, false, 0.0, 0.0, timed, 0, 0);
+        rc = launch_update(c, c->main_begin, c->main_items, A, B, B, false, fixed, dt, dt, timed, 0, 0);
         if (rc) return rc;
-        for (size_t r = 0; r < c->fix_rounds.size(); r++) {
+        for (size_t r = 0; !fixed && r < c->fix_rounds.size(); r++) {
             rc = launch_update(c, c->fix_rounds[r].first, c->fix_rounds[r].second, A, B, B, false, false, 0.0, 0.0,
                                timed, (int)r + 1, (int)r + 1);
             if (rc) return rc;
@@ -867,11 +811,8 @@ int hf_advance(hf_ctx* c, const double* dt)
     return HF_OK;
 }
 
-int hf_run(hf_ctx* c, int nsteps, int flags)
+static int run_loop(hf_ctx* c, int nsteps, int flags, bool fixed, double dt)
 {
-    if (!c) return fail(HF_EINVAL, "null context");
-    if (nsteps < 0) return fail(HF_EINVAL, "hf_run: nsteps must be >= 0");
-    if (nsteps == 0) return HF_OK;
     int rc = ensure_partials(c);
     if (rc) return rc;
     HF_HIP(hipEventSynchronize(c->calls_copied));   // earlier upload must have consumed the staging buffer
@@ -879,13 +820,15 @@ int hf_run(hf_ctx* c, int nsteps, int flags)
     HF_HIP(hipMemcpyAsync(c->d_calls, c->h_calls_pinned, sizeof(uint32_t) * c->np, hipMemcpyHostToDevice, c->stream));
     HF_HIP(hipEventRecord(c->calls_copied, c->stream));
     if (flags & HF_RUN_GRAPH) {
-        const auto key = std::make_pair(nsteps, c->cur);
+        uint64_t dtbits = 0;
+        if (fixed) std::memcpy(&dtbits, &dt, sizeof dtbits);
+        const auto key = std::make_tuple(nsteps, c->cur, dtbits);
         auto it = c->graphs.find(key);
         if (it == c->graphs.end()) {
             hipGraph_t g = nullptr;
             hipGraphExec_t ge = nullptr;
             HF_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            rc = enqueue_steps(c, nsteps, false);
+            rc = enqueue_steps(c, nsteps, false, fixed, dt);
             hipError_t ce = hipStreamEndCapture(c->stream, &g);
             if (rc) {
                 if (g) hipGraphDestroy(g);
@@ -897,13 +840,31 @@ int hf_run(hf_ctx* c, int nsteps, int flags)
         }
         HF_HIP(hipGraphLaunch(it->second.second, c->stream));
     } else {
-        rc = enqueue_steps(c, nsteps, c->profile);
+        rc = enqueue_steps(c, nsteps, c->profile, fixed, dt);
         if (rc) return rc;
     }
     c->cur ^= (nsteps & 1);
-    for (int p = 0; p < c->np; p++) c->calls[p] += (int64_t)nsteps * c->probs[p].cps;
+    if (!fixed)
+        for (int p = 0; p < c->np; p++) c->calls[p] += (int64_t)nsteps * c->probs[p].cps;
     c->ghosts_pending = false;
     return HF_OK;
+}
+
+int hf_run(hf_ctx* c, int nsteps, int flags)
+{
+    if (!c) return fail(HF_EINVAL, "null context");
+    if (nsteps < 0) return fail(HF_EINVAL, "hf_run: nsteps must be >= 0");
+    if (nsteps == 0) return HF_OK;
+    return run_loop(c, nsteps, flags, false, 0.0);
+}
+
+int hf_run_fixed(hf_ctx* c, int nsteps, double dt, int flags)
+{
+    if (!c) return fail(HF_EINVAL, "null context");
+    if (nsteps < 0) return fail(HF_EINVAL, "hf_run_fixed: nsteps must be >= 0");
+    if (!(dt > 0.0) || !std::isfinite(dt)) return fail(HF_EINVAL, "hf_run_fixed: dt must be finite and > 0");
+    if (nsteps == 0) return HF_OK;
+    return run_loop(c, nsteps, flags, true, dt);
 }
 
 int hf_synchronize(hf_ctx* c)

@@ -24,6 +24,11 @@ HF_RUN_GRAPH = 1
 EXPORTS = ("hf_create", "hf_destroy", "hf_ghost_cells", "hf_upload", "hf_download", "hf_apply_bcs",
            "hf_time_step", "hf_scheme_residual", "hf_advance", "hf_run", "hf_synchronize", "hf_dt_history",
            "hf_calls", "hf_profile", "hf_kernel_stats", "hf_conservation", "hf_last_error", "hf_version")
+# include/hyperflow_learned.h
+EXPORTS_LEARNED = ("hf_net_create", "hf_net_destroy", "hf_net_set_stream", "hf_net_set_desc", "hf_net_param_count",
+                   "hf_net_set_params", "hf_net_get_params", "hf_net_reset_optimizer", "hf_net_forward",
+                   "hf_net_gradients", "hf_net_train_step", "hf_net_synchronize", "hf_correct", "hf_run_fixed",
+                   "hf_state_to_device", "hf_state_from_device", "hf_restrict2", "hf_stream")
 
 
 class hf_block_desc(C.Structure):
@@ -36,6 +41,11 @@ class hf_case_desc(C.Structure):
     _fields_ = [("scheme", C.c_int32), ("limiter", C.c_int32), ("precision", C.c_int32), ("dt_mode", C.c_int32),
                 ("gamma", C.c_double), ("cfl", C.c_double), ("nblocks", C.c_int32), ("tile_rows", C.c_int32),
                 ("blocks", C.POINTER(hf_block_desc))]
+
+
+class hf_net_desc(C.Structure):
+    _fields_ = [("in_mean", C.c_float * 4), ("in_scale", C.c_float * 4), ("out_scale", C.c_float * 4),
+                ("gamma", C.c_float), ("shock_threshold", C.c_float), ("seed", C.c_uint64)]
 
 
 class HyperFlowError(RuntimeError):
@@ -74,6 +84,28 @@ def lib():
         L.hf_last_error.argtypes = []
         L.hf_version.restype = C.c_char_p
         L.hf_version.argtypes = []
+        _fp = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+        L.hf_net_create.argtypes = [C.POINTER(hf_net_desc), C.POINTER(vp)]
+        L.hf_net_destroy.argtypes = [vp]
+        L.hf_net_destroy.restype = None
+        L.hf_net_set_stream.argtypes = [vp, vp]
+        L.hf_net_set_desc.argtypes = [vp, C.POINTER(hf_net_desc)]
+        L.hf_net_param_count.argtypes = []
+        L.hf_net_param_count.restype = C.c_int64
+        L.hf_net_set_params.argtypes = [vp, _fp, C.c_size_t]
+        L.hf_net_get_params.argtypes = [vp, _fp, C.c_size_t]
+        L.hf_net_reset_optimizer.argtypes = [vp]
+        L.hf_net_forward.argtypes = [vp, vp, C.c_int, C.c_int, vp]
+        L.hf_net_gradients.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp, vp]
+        L.hf_net_train_step.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_float, vp]
+        L.hf_net_synchronize.argtypes = [vp]
+        L.hf_correct.argtypes = [vp, vp, C.POINTER(C.c_int64)]
+        L.hf_run_fixed.argtypes = [vp, C.c_int, C.c_double, C.c_int]
+        L.hf_state_to_device.argtypes = [vp, C.c_int, vp]
+        L.hf_state_from_device.argtypes = [vp, C.c_int, vp]
+        L.hf_restrict2.argtypes = [vp, C.c_int, vp]
+        L.hf_stream.argtypes = [vp]
+        L.hf_stream.restype = vp
         _lib = L
     return _lib
 
@@ -225,6 +257,128 @@ class Solver:
         out = np.zeros(4, np.float64)
         _check(lib().hf_conservation(self._h, problem, out), "hf_conservation")
         return out
+
+    # ------------------------------------------- fixed dt, device transfers --
+    def run_fixed(self, nsteps: int, dt: float, graph: bool = False) -> None:
+        """nsteps steps with a fixed dt (update and MUSCL predictor)."""
+        _check(lib().hf_run_fixed(self._h, int(nsteps), float(dt), HF_RUN_GRAPH if graph else 0), "hf_run_fixed")
+
+    def state_to_device(self, b: int, dst) -> None:
+        """Interior of block b into dense FP32 planes [4][ny][nx] at dst (a device tensor)."""
+        _check(lib().hf_state_to_device(self._h, b, _ptr(dst)), "hf_state_to_device")
+
+    def state_from_device(self, b: int, src) -> None:
+        _check(lib().hf_state_from_device(self._h, b, _ptr(src)), "hf_state_from_device")
+
+    def restrict2(self, b: int, dst) -> None:
+        """2x2 block average of block b's interior into dense FP32 planes [4][ny/2][nx/2]."""
+        _check(lib().hf_restrict2(self._h, b, _ptr(dst)), "hf_restrict2")
+
+    def correct(self, net: "Net", count: bool = False) -> Optional[int]:
+        """Learned correction of the current state in place; with count, the
+        number of cells left at the plain state (synchronises)."""
+        if count:
+            n = C.c_int64()
+            _check(lib().hf_correct(self._h, net._h, C.byref(n)), "hf_correct")
+            return n.value
+        _check(lib().hf_correct(self._h, net._h, None), "hf_correct")
+        return None
+
+    @property
+    def stream(self) -> int:
+        return lib().hf_stream(self._h) or 0
+
+
+def _ptr(t) -> int:
+    """Device address of a tensor (anything with data_ptr()) or an int."""
+    if hasattr(t, "data_ptr"):
+        if hasattr(t, "is_contiguous") and not t.is_contiguous():
+            raise ValueError("device buffer must be contiguous")
+        if hasattr(t, "dtype") and str(t.dtype) != "torch.float32":
+            raise ValueError("device buffer must be float32")
+        return t.data_ptr()
+    return int(t)
+
+
+class Net:
+    """The learned-correction CNN (include/hyperflow_learned.h)."""
+
+    def __init__(self, in_mean=(0.0,) * 4, in_scale=(1.0,) * 4, out_scale=(1.0,) * 4, gamma: float = 1.4,
+                 shock_threshold: float = 0.0, seed: int = 0):
+        self._desc = self._make_desc(in_mean, in_scale, out_scale, gamma, shock_threshold, seed)
+        h = C.c_void_p()
+        _check(lib().hf_net_create(C.byref(self._desc), C.byref(h)), "hf_net_create")
+        self._h = h
+
+    @staticmethod
+    def _make_desc(in_mean, in_scale, out_scale, gamma, shock_threshold, seed=0):
+        d = hf_net_desc()
+        for r in range(4):
+            d.in_mean[r] = float(in_mean[r])
+            d.in_scale[r] = float(in_scale[r])
+            d.out_scale[r] = float(out_scale[r])
+        d.gamma = float(gamma)
+        d.shock_threshold = float(shock_threshold)
+        d.seed = int(seed)
+        return d
+
+    def set_desc(self, in_mean, in_scale, out_scale, gamma: float = 1.4, shock_threshold: float = 0.0) -> None:
+        self._desc = self._make_desc(in_mean, in_scale, out_scale, gamma, shock_threshold)
+        _check(lib().hf_net_set_desc(self._h, C.byref(self._desc)), "hf_net_set_desc")
+
+    @property
+    def desc(self) -> dict:
+        d = self._desc
+        return {"in_mean": list(d.in_mean), "in_scale": list(d.in_scale), "out_scale": list(d.out_scale),
+                "gamma": d.gamma, "shock_threshold": d.shock_threshold}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().hf_net_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def param_count() -> int:
+        return int(lib().hf_net_param_count())
+
+    def set_stream(self, stream: int) -> None:
+        _check(lib().hf_net_set_stream(self._h, C.c_void_p(stream or None)), "hf_net_set_stream")
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        _check(lib().hf_net_set_params(self._h, p, p.size), "hf_net_set_params")
+
+    def get_params(self) -> np.ndarray:
+        out = np.empty(self.param_count(), np.float32)
+        _check(lib().hf_net_get_params(self._h, out, out.size), "hf_net_get_params")
+        return out
+
+    def reset_optimizer(self) -> None:
+        _check(lib().hf_net_reset_optimizer(self._h), "hf_net_reset_optimizer")
+
+    def forward(self, u, y) -> None:
+        """y = net(normalised u); u, y: FP32 device tensors [4][ny][nx]."""
+        _check(lib().hf_net_forward(self._h, _ptr(u), u.shape[2], u.shape[1], _ptr(y)), "hf_net_forward")
+
+    def gradients(self, u, target):
+        g = np.empty(self.param_count(), np.float32)
+        loss = np.zeros(1, np.float32)
+        _check(lib().hf_net_gradients(self._h, _ptr(u), _ptr(target), u.shape[2], u.shape[1], g.ctypes.data,
+                                      loss.ctypes.data), "hf_net_gradients")
+        return float(loss[0]), g
+
+    def train_step(self, u, target, lr: float, loss_out=None) -> None:
+        _check(lib().hf_net_train_step(self._h, _ptr(u), _ptr(target), u.shape[2], u.shape[1], float(lr),
+                                       _ptr(loss_out) if loss_out is not None else None), "hf_net_train_step")
+
+    def synchronize(self) -> None:
+        _check(lib().hf_net_synchronize(self._h), "hf_net_synchronize")
 
 
 def version() -> str:

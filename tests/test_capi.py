@@ -1,5 +1,5 @@
-"""The C-ABI library: loads, exports every entry point of include/hyperflow.h,
-and validates case descriptors on the host (no GPU work is issued here)."""
+"""The C-ABI library: loads, exports every entry point of include/*.h, and
+validates case descriptors on the host (no GPU work is issued here)."""
 import ctypes as C
 import os
 import re
@@ -12,8 +12,8 @@ from hyperflow_amd import case as K
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def header_functions():
-    text = open(os.path.join(ROOT, "include", "hyperflow.h")).read()
+def header_functions(name="hyperflow.h"):
+    text = open(os.path.join(ROOT, "include", name)).read()
     return sorted(set(re.findall(r"HF_API\s+[\w\s\*]+?\b(hf_\w+)\s*\(", text)))
 
 
@@ -24,6 +24,17 @@ def test_library_exports_every_declared_entry_point():
     for n in names:
         assert hasattr(lib, n), n
     assert set(names) == set(native.EXPORTS)
+    learned = header_functions("hyperflow_learned.h")
+    for n in learned:
+        assert hasattr(lib, n), n
+    assert set(learned) == set(native.EXPORTS_LEARNED)
+    headers = sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h"))
+    assert headers == ["hyperflow.h", "hyperflow_learned.h"]
+
+
+def test_net_desc_layout_and_param_count():
+    assert C.sizeof(native.hf_net_desc) == 14 * 4 + 8
+    assert native.Net.param_count() == 9 * 4 * 32 + 32 + 3 * (9 * 32 * 32 + 32) + 9 * 32 * 4 + 4
 
 
 def test_struct_layout_matches_header():

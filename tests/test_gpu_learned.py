@@ -1,0 +1,189 @@
+"""Learned coarse-grid correction on the GPU: the BF16 MFMA convolution
+kernels (forward, data and weight gradients) against the CPU reference, a
+training overfit check, the solver transfers (restriction, state copies,
+fixed-dt steps) and the corrected step with its fallback."""
+import numpy as np
+import pytest
+
+import cnn_ref as R
+import hfo
+from gpu_util import say
+from hyperflow_amd import case as C
+from hyperflow_amd import native
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+NORM = dict(in_mean=[1.0, 0.0, 0.0, 2.5], in_scale=[1.5, 1.0, 1.0, 0.8], out_scale=[0.02, 0.02, 0.02, 0.05])
+OFFS = np.cumsum([0] + [9 * ci * co + co for ci, co in R.LAYERS])
+
+
+def dev(a):
+    return torch.tensor(np.ascontiguousarray(a, np.float32), device="cuda")
+
+
+def state(seed, ny, nx, gamma=1.4):
+    """A smooth random physical state, conserved planes [4][ny][nx]."""
+    rng = np.random.default_rng(seed)
+    yy, xx = np.meshgrid(np.linspace(0, 1, ny), np.linspace(0, 1, nx), indexing="ij")
+    k = rng.uniform(1, 4, size=4)
+    rho = 1.0 + 0.5 * np.sin(2 * np.pi * k[0] * xx) * np.cos(2 * np.pi * k[1] * yy)
+    u = 0.5 * np.cos(2 * np.pi * k[2] * yy)
+    v = 0.5 * np.sin(2 * np.pi * k[3] * xx)
+    p = 1.0 + 0.3 * np.cos(2 * np.pi * (xx + yy))
+    E = p / (gamma - 1) + 0.5 * rho * (u * u + v * v)
+    return np.stack([rho, rho * u, rho * v, E])
+
+
+def test_forward_matches_cpu_reference():
+    net = native.Net(**NORM, seed=7)
+    p = net.get_params()
+    u = state(1, 48, 64).astype(np.float32)
+    y = torch.zeros((4, 48, 64), device="cuda")
+    say("learned: forward 64x48")
+    net.forward(dev(u), y)
+    net.synchronize()
+    torch.cuda.synchronize()
+    yg = y.cpu().numpy().astype(np.float64)
+    yr, _ = R.forward(p, u, NORM["in_mean"], NORM["in_scale"], emulate=True)
+    err = np.max(np.abs(yg - yr)) / np.max(np.abs(yr))
+    mean_err = np.mean(np.abs(yg - yr)) / np.mean(np.abs(yr))
+    say(f"  max rel {err:.2e}, mean rel {mean_err:.2e}")
+    assert err < 2e-3 and mean_err < 2e-4
+
+
+def test_gradients_match_cpu_reference():
+    net = native.Net(**NORM, seed=8)
+    p = net.get_params()
+    u = state(2, 40, 64).astype(np.float32)
+    t = (u + np.random.default_rng(3).normal(scale=0.01, size=u.shape)).astype(np.float32)
+    say("learned: gradients 64x40")
+    L, g = net.gradients(dev(u), dev(t))
+    Lr, gr = R.gradients(p, u, t, emulate=True, **NORM)
+    _, g64 = R.gradients(p, u, t, emulate=False, **NORM)
+    say(f"  loss gpu {L:.6e} ref {Lr:.6e}")
+    assert abs(L - Lr) <= 1e-4 * Lr
+    for l in range(5):
+        a, b = slice(OFFS[l], OFFS[l + 1]), slice(OFFS[l], OFFS[l + 1])
+        rel = np.linalg.norm(g[a] - gr[b]) / np.linalg.norm(gr[b])
+        rel64 = np.linalg.norm(g[a] - g64[b]) / np.linalg.norm(g64[b])
+        say(f"  layer {l + 1}: rel err vs emulated {rel:.2e}, vs exact {rel64:.2e}")
+        assert rel < 1e-2
+        assert rel64 < 0.2
+
+
+def test_training_overfits_one_pair():
+    """A target that is a local function of the input (a scaled 5-point
+    Laplacian of the state) is learned to a tenth of the initial loss."""
+    net = native.Net(**NORM, seed=9)
+    u0 = state(4, 64, 64)
+    up = np.pad(u0, ((0, 0), (1, 1), (1, 1)), mode="edge")
+    lap = up[:, :-2, 1:-1] + up[:, 2:, 1:-1] + up[:, 1:-1, :-2] + up[:, 1:-1, 2:] - 4 * u0
+    rng = np.random.default_rng(5)
+    u = dev(u0)
+    t = dev(u0 + 0.2 * lap + 1e-4 * rng.normal(size=u0.shape))
+    say("learned: overfit 64x64")
+    L0, _ = net.gradients(u, t)
+    for _ in range(300):
+        net.train_step(u, t, 2e-3)
+    L1, _ = net.gradients(u, t)
+    say(f"  loss {L0:.4e} -> {L1:.4e}")
+    assert L1 < 0.1 * L0
+
+
+def _fp32_solver(nx=128, ny=64, seed=3, scheme="MUSCLHancock"):
+    case = C.quad_case(nx, ny, seed, scheme=scheme, limiter="VanLeer", cfl=0.5)
+    s = native.Solver(case, precision="fp32", dt_mode="uniform")
+    f = C.perturbed_fields(case, seed)
+    s.upload(0, f[0])
+    return case, s, f[0]
+
+
+def test_state_transfers_and_restriction():
+    case, s, f = _fp32_solver()
+    g = case.ghost
+    s.run(3)
+    s.synchronize()
+    full = s.download(0)[:, g:-g, g:-g].astype(np.float32)
+    d = torch.zeros((4, 64, 128), device="cuda")
+    c = torch.zeros((4, 32, 64), device="cuda")
+    s.state_to_device(0, d)
+    s.restrict2(0, c)
+    s.synchronize()
+    np.testing.assert_array_equal(d.cpu().numpy(), full)
+    q = 0.25 * ((full[:, 0::2, 0::2] + full[:, 0::2, 1::2]) + (full[:, 1::2, 0::2] + full[:, 1::2, 1::2]))
+    np.testing.assert_array_equal(c.cpu().numpy(), q.astype(np.float32))
+    np.testing.assert_allclose(4.0 * q.astype(np.float64).sum(axis=(1, 2)), full.astype(np.float64).sum(axis=(1, 2)),
+                               rtol=1e-6)
+    mod = full * np.float32(1.5)
+    torch.cuda.synchronize()
+    s.state_from_device(0, dev(mod))
+    s.synchronize()
+    np.testing.assert_array_equal(s.download(0)[:, g:-g, g:-g].astype(np.float32), mod)
+
+
+@pytest.mark.parametrize("scheme", ["Godunov", "MUSCLHancock"])
+def test_fixed_dt_step_matches_oracle(scheme):
+    """One FP64 step with dt fixed to the oracle's own first dt equals the
+    oracle's step bit for bit (single block: the MUSCL predictor's TimeStep
+    call sees the same mesh, so its dt is the step dt)."""
+    case = C.quad_case(64, 48, 11, scheme=scheme, limiter="MinBee", cfl=0.5)
+    f = case.initial_fields()
+    hfo.set_threads(4)
+    m = hfo.OracleMesh(case, f)
+    dt = m.step()
+    s = native.Solver(case, precision="fp64", dt_mode="uniform")
+    s.upload(0, f[0])
+    s.run_fixed(1, dt)
+    s.synchronize()
+    g = case.ghost
+    np.testing.assert_array_equal(s.download(0)[:, g:-g, g:-g], m.fields(0)[:, g:-g, g:-g])
+
+
+def test_fixed_dt_graph_equals_eager():
+    _, a, f = _fp32_solver(seed=5)
+    _, b, _ = _fp32_solver(seed=5)
+    a.run_fixed(6, 1e-4)
+    for _ in range(2):
+        b.run_fixed(3, 1e-4, graph=True)
+    a.synchronize()
+    b.synchronize()
+    np.testing.assert_array_equal(a.download(0), b.download(0))
+
+
+def test_correct_matches_cpu_apply_and_fallback():
+    """The in-place corrected state equals the CPU application of the same
+    network output in FP32, cell by cell, including the fallback mask."""
+    case, s, f = _fp32_solver(seed=6)
+    g = case.ghost
+    net = native.Net(in_mean=[1, 0, 0, 2.5], in_scale=[1, 1, 1, 1], out_scale=[0.5, 0.1, 0.1, 1.0], gamma=1.4,
+                     shock_threshold=0.2, seed=12)
+    p = net.get_params()
+    p[OFFS[4] + 9 * 32 * 4 + 0] = -0.3      # push the density down so some cells go non-positive
+    net.set_params(p)
+    s.run(2)
+    s.synchronize()
+    U = s.download(0)[:, g:-g, g:-g].astype(np.float32)
+    y = torch.zeros((4, 64, 128), device="cuda")
+    net.forward(dev(U), y)
+    net.synchronize()
+    yv = y.cpu().numpy()
+    nfb = s.correct(net, count=True)
+    out = s.download(0)[:, g:-g, g:-g].astype(np.float32)
+    exp, bad = R.apply_correction(U, yv, net.desc["out_scale"], 1.4, 0.2, dtype=np.float32)
+    say(f"learned: fallback {nfb} of {U[0].size} cells (cpu {int(bad.sum())})")
+    assert nfb == int(bad.sum())
+    assert 0 < nfb < U[0].size
+    np.testing.assert_array_equal(out, exp.astype(np.float32))
+
+
+def test_correct_all_fallback_leaves_state():
+    case, s, f = _fp32_solver(seed=7)
+    net = native.Net(**NORM, seed=1)
+    p = np.zeros(native.Net.param_count(), np.float32)
+    p[OFFS[4] + 9 * 32 * 4 + 0] = -1e4       # density correction -200 everywhere
+    net.set_params(p)
+    before = s.download(0)
+    nfb = s.correct(net, count=True)
+    assert nfb == 128 * 64
+    np.testing.assert_array_equal(s.download(0), before)
