@@ -14,6 +14,9 @@ Other workloads of the same path (not the headline line):
     --workload c4   512 independent problems of U[64,512]^2 cells, MUSCL FP32,
                     one persistent launch per step, HIP graph; also times
                     looping the same problems one context at a time
+    --workload c5   learned coarse correction: 1024^2 MUSCL FP32 + the committed
+                    CNN against plain runs at 1024..1792^2 and the 2048^2
+                    solution (density L1, wall time, speed-up at equal L1)
     --workload copy a 1 GiB device copy: measured copy bandwidth, and the
                     calibration run for the HBM traffic counters
 """
@@ -84,6 +87,31 @@ def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 20):
             "launches": reps + 3}
 
 
+def learned_line(seeds=(5000, 5001, 5002), coarse_steps: int = 200):
+    """The learned-correction comparison on unseen seeds: density L1 against
+    the 2x2-averaged 2048^2 solution, and wall times (HIP graphs, state in
+    HBM), for plain runs at 1024..1792^2 and the corrected 1024^2 run."""
+    from hyperflow_amd import learned as L
+    L.ensure_tuned((1024, 1280, 1536, 1792, 2048), log=lambda m: print(m, file=sys.stderr))
+    net = L.load()
+    meta = L.load_meta()
+    every, start = int(meta["every"]), int(meta["start"])
+    rows, summ = L.evaluate(net, list(seeds), coarse_steps, every=(every,), start=start,
+                            log=lambda m: print(m, file=sys.stderr))
+    lr = summ["learned"][every]
+    return {"metric": "learned vs plain time at equal L1", "value": round(lr["speedup_at_equal_l1"], 4), "unit": "x",
+            "higher_is_better": True, "n_gpus": 1, "dtype": "f32 (solver), bf16/f32 (network)",
+            "data": "synthetic (seeded quadrant states with sinusoidal density perturbation)",
+            "config": {"workload": "MUSCL-Hancock+VanLeer+HLLC FP32 on [0,1]^2, transmissive, fixed Courant 0.4/6.6; "
+                                   f"learned 1024^2 corrected every {every} steps from step {start} vs plain n^2",
+                       "coarse_steps": coarse_steps, "seeds": list(seeds)},
+            "learned_l1": lr["l1"], "learned_seconds": lr["seconds"], "plain_resolutions": summ["resolutions"],
+            "plain_l1": summ["plain_l1"], "plain_seconds": summ["plain_seconds"],
+            "plain_2048_seconds": summ["fine_seconds"], "plain_n_at_equal_l1": lr["plain_n_at_equal_l1"],
+            "l1_gain_vs_plain_1024": lr["l1_gain_vs_plain_1024"], "speedup_vs_plain_2048": lr["speedup_vs_plain_2048"],
+            "vs_baseline": None}
+
+
 def traffic_from_profiles(workload: str):
     """HBM bytes per update launch measured with rocprofv3 --pmc (FETCH_SIZE and
     WRITE_SIZE in separate runs, each scaled by what the counter reads for a
@@ -150,7 +178,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "copy"], default="c2")
+    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "c5", "copy"], default="c2")
     ap.add_argument("--n", type=int, default=1024, help="cells per block side (c2), grid side (c3)")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default=None)
     ap.add_argument("--tile-rows", type=int, default=None, help="default: autotuned value if cached")
@@ -165,6 +193,9 @@ def main():
         sys.exit(2)
     if args.workload == "copy":
         print(json.dumps(copy_bandwidth()), flush=True)
+        return
+    if args.workload == "c5":
+        print(json.dumps(learned_line()), flush=True)
         return
 
     from hyperflow_amd import autotune, native

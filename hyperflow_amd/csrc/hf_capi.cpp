@@ -813,7 +813,8 @@ int hf_advance(hf_ctx* c, const double* dt)
 
 static int run_loop(hf_ctx* c, int nsteps, int flags, bool fixed, double dt)
 {
-    int rc = ensure_partials(c);
+    // fixed-dt steps never read the wave-speed maxima
+    int rc = fixed ? HF_OK : ensure_partials(c);
     if (rc) return rc;
     HF_HIP(hipEventSynchronize(c->calls_copied));   // earlier upload must have consumed the staging buffer
     for (int p = 0; p < c->np; p++) c->h_calls_pinned[p] = (uint32_t)c->calls[p];
@@ -846,6 +847,8 @@ static int run_loop(hf_ctx* c, int nsteps, int flags, bool fixed, double dt)
     c->cur ^= (nsteps & 1);
     if (!fixed)
         for (int p = 0; p < c->np; p++) c->calls[p] += (int64_t)nsteps * c->probs[p].cps;
+    else
+        c->partials_valid = false;
     c->ghosts_pending = false;
     return HF_OK;
 }

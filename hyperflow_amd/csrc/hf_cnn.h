@@ -36,6 +36,7 @@ struct ConvArgs {
     int32_t pitch;          // APPLY: row pitch of U
     float gamma;            // APPLY: ideal-gas gamma for the pressure check
     float scale[4];         // APPLY: correction = scale * network output
+    float max_output;       // APPLY: fall back where |output| > max_output (<= 0: no bound)
     const uint8_t* shock;   // APPLY: per-cell shock flag [ny][nx] (null: none)
     uint32_t* fallback;     // APPLY: cells left at the plain state
 };

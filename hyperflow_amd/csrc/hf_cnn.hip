@@ -71,29 +71,49 @@ __global__ __launch_bounds__(256) void k_conv(const ConvArgs a)
         }
 
     uint32_t nfb = 0;
+    // CIN = 32: the B operand of tap (dy, dx) at row y is input row y+dy at
+    // column x+dx, so a window of three rows x three column shifts slides down
+    // with y and each new row costs three loads, issued one row ahead.
+    bf16x8 Bw[3][3], Bn[3];
+    if (CIN == 32) {
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+                Bw[r][d] = load8(a.in + ((size_t)(y0 + r) * W2 + (x + d)) * CIN + g * 8);
+    }
     for (int y = y0; y < y1; y++) {
         f32x4 acc[MT];
 #pragma unroll
         for (int mt = 0; mt < MT; mt++) acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        if (CIN == 32) {
+            const int yn = min(y + 3, a.ny + 1);   // padded row of input row y+2
 #pragma unroll
-        for (int s = 0; s < S; s++) {
-            int tap, cb;
-            bool valid = true;
-            if (CIN == 32) {
-                tap = s;
-                cb = g;
-            } else {
-                tap = 4 * s + g;
-                cb = 0;
-                valid = tap < 9;
-                tap = valid ? tap : 8;
+            for (int d = 0; d < 3; d++) Bn[d] = load8(a.in + ((size_t)yn * W2 + (x + d)) * CIN + g * 8);
+#pragma unroll
+            for (int t = 0; t < 9; t++)
+#pragma unroll
+                for (int mt = 0; mt < MT; mt++)
+                    acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wA[mt][t], Bw[t / 3][t % 3], acc[mt], 0, 0, 0);
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                Bw[0][d] = Bw[1][d];
+                Bw[1][d] = Bw[2][d];
+                Bw[2][d] = Bn[d];
             }
-            const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-            bf16x8 B = load8(a.in + ((size_t)(y + 1 + dy) * W2 + (x + 1 + dx)) * CIN + cb * 8);
-            if (CIN != 32) B = valid ? B : zero8();
+        } else {
 #pragma unroll
-            for (int mt = 0; mt < MT; mt++)
-                acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wA[mt][s], B, acc[mt], 0, 0, 0);
+            for (int s = 0; s < S; s++) {
+                int tap = 4 * s + g;
+                const bool valid = tap < 9;
+                tap = valid ? tap : 8;
+                const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+                bf16x8 B = load8(a.in + ((size_t)(y + 1 + dy) * W2 + (x + 1 + dx)) * CIN);
+                B = valid ? B : zero8();
+#pragma unroll
+                for (int mt = 0; mt < MT; mt++)
+                    acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wA[mt][s], B, acc[mt], 0, 0, 0);
+            }
         }
 
         const size_t pix = (size_t)(y + 1) * W2 + (x + 1);
@@ -137,7 +157,11 @@ __global__ __launch_bounds__(256) void k_conv(const ConvArgs a)
                 }
                 const float p = (a.gamma - 1.0f) * (n[3] - 0.5f * (n[1] * n[1] + n[2] * n[2]) / n[0]);
                 const bool shock = a.shock && a.shock[(size_t)y * a.nx + x];
-                const bool ok = n[0] > 0.0f && p > 0.0f && !shock && isfinite(p) && isfinite(n[1]) &&
+                bool inrange = true;
+                if (a.max_output > 0.0f)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) inrange = inrange && fabsf(v[r]) <= a.max_output;
+                const bool ok = n[0] > 0.0f && p > 0.0f && !shock && inrange && isfinite(p) && isfinite(n[1]) &&
                                 isfinite(n[2]) && isfinite(n[3]);
                 if (ok) {
 #pragma unroll
@@ -325,25 +349,35 @@ __global__ __launch_bounds__(256) void k_loss(const LossArgs a)
     if (threadIdx.x == 0) a.part[blockIdx.x] = red[0];
 }
 
-// Sum of the workgroup partials in workgroup order; output in the layer's
-// real layout W[9][ci_r][co_r] followed by b[co_r].
+// Sum of the workgroup partials: 64 consecutive partial entries per block,
+// four interleaved partial subsets per entry combined in a fixed order, so
+// the result is deterministic.  Output in the layer's real layout
+// W[9][ci_r][co_r] followed by b[co_r] (padding channels dropped).
 __global__ __launch_bounds__(256) void k_reduce_grad(const float* __restrict__ part, int nparts, int CI, int CO,
                                                      int ci_r, int co_r, float* __restrict__ gout)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    const int nw = 9 * ci_r * co_r;
-    if (i >= nw + co_r) return;
-    int src;
-    if (i < nw) {
-        const int tap = i / (ci_r * co_r), rem = i % (ci_r * co_r);
-        src = (tap * CI + rem / co_r) * CO + rem % co_r;
-    } else {
-        src = 9 * CI * CO + (i - nw);
+    __shared__ float red[4][64];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int stride = 9 * CI * CO + CO;
+    const int s = blockIdx.x * 64 + tx;
+    float acc = 0.0f;
+    if (s < stride)
+        for (int w = ty; w < nparts; w += 4) acc += part[(size_t)w * stride + s];
+    red[ty][tx] = acc;
+    __syncthreads();
+    if (ty == 0 && s < stride) {
+        const float v = (red[0][tx] + red[1][tx]) + (red[2][tx] + red[3][tx]);
+        int dst = -1;
+        if (s < 9 * CI * CO) {
+            const int tap = s / (CI * CO), rem = s % (CI * CO);
+            const int ci = rem / CO, co = rem % CO;
+            if (ci < ci_r && co < co_r) dst = (tap * ci_r + ci) * co_r + co;
+        } else {
+            const int co = s - 9 * CI * CO;
+            if (co < co_r) dst = 9 * ci_r * co_r + co;
+        }
+        if (dst >= 0) gout[dst] = v;
     }
-    const size_t stride = (size_t)9 * CI * CO + CO;
-    float s = 0.0f;
-    for (int w = 0; w < nparts; w++) s += part[(size_t)w * stride + src];
-    gout[i] = s;
 }
 
 __global__ __launch_bounds__(256) void k_reduce_sum(const float* __restrict__ part, int n, float scale,
@@ -482,8 +516,8 @@ hipError_t launch_loss(const LossArgs& a, int* nblocks, hipStream_t s)
 hipError_t launch_reduce_grad(const float* part, int nparts, int ci, int co, int ci_r, int co_r, float* g,
                               hipStream_t s)
 {
-    const int n = 9 * ci_r * co_r + co_r;
-    hipLaunchKernelGGL(k_reduce_grad, dim3((n + 255) / 256), dim3(256), 0, s, part, nparts, ci, co, ci_r, co_r, g);
+    const int n = 9 * ci * co + co;
+    hipLaunchKernelGGL(k_reduce_grad, dim3((n + 63) / 64), dim3(256), 0, s, part, nparts, ci, co, ci_r, co_r, g);
     return hipGetLastError();
 }
 

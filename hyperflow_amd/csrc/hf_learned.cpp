@@ -336,6 +336,8 @@ int hf_net_set_desc(hf_net* n, const hf_net_desc* d)
         if (!(d->out_scale[r] != 0.0f) || !std::isfinite(d->out_scale[r]) || !std::isfinite(d->in_scale[r]) ||
             !std::isfinite(d->in_mean[r]))
             return fail(HF_EINVAL, "hf_net_set_desc: scales must be finite, out_scale non-zero");
+    if (!std::isfinite(d->max_output) || !std::isfinite(d->shock_threshold) || !std::isfinite(d->gamma))
+        return fail(HF_EINVAL, "hf_net_set_desc: gamma, shock_threshold and max_output must be finite");
     const uint64_t seed = n->d.seed;
     n->d = *d;
     n->d.seed = seed;
@@ -477,6 +479,7 @@ int hf_correct(hf_ctx* c, hf_net* n, int64_t* fallback)
         a.pitch = B.pitch;
         a.gamma = n->d.gamma;
         for (int r = 0; r < 4; r++) a.scale[r] = n->d.out_scale[r];
+        a.max_output = n->d.max_output;
         a.shock = sensor ? n->shock : nullptr;
         a.fallback = n->fallback;
         HF_HIP(hfn::launch_conv(32, 8, hfn::CONV_APPLY, a, c->stream));

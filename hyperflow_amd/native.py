@@ -3,6 +3,11 @@
 The library is the product: every call here goes to the HIP kernels.  There is
 no CPU fallback -- if the library is missing or no GPU is present the calls
 fail loudly.
+
+A process that also uses torch on the GPU must import torch before the first
+call here: torch bundles its own HIP runtime with the same soname as ROCm's,
+and the process runs on whichever is loaded first (torch does not initialise
+on another one).
 """
 from __future__ import annotations
 
@@ -45,7 +50,8 @@ class hf_case_desc(C.Structure):
 
 class hf_net_desc(C.Structure):
     _fields_ = [("in_mean", C.c_float * 4), ("in_scale", C.c_float * 4), ("out_scale", C.c_float * 4),
-                ("gamma", C.c_float), ("shock_threshold", C.c_float), ("seed", C.c_uint64)]
+                ("gamma", C.c_float), ("shock_threshold", C.c_float), ("max_output", C.c_float),
+                ("reserved", C.c_float), ("seed", C.c_uint64)]
 
 
 class HyperFlowError(RuntimeError):
@@ -304,14 +310,14 @@ class Net:
     """The learned-correction CNN (include/hyperflow_learned.h)."""
 
     def __init__(self, in_mean=(0.0,) * 4, in_scale=(1.0,) * 4, out_scale=(1.0,) * 4, gamma: float = 1.4,
-                 shock_threshold: float = 0.0, seed: int = 0):
-        self._desc = self._make_desc(in_mean, in_scale, out_scale, gamma, shock_threshold, seed)
+                 shock_threshold: float = 0.0, seed: int = 0, max_output: float = 0.0):
+        self._desc = self._make_desc(in_mean, in_scale, out_scale, gamma, shock_threshold, seed, max_output)
         h = C.c_void_p()
         _check(lib().hf_net_create(C.byref(self._desc), C.byref(h)), "hf_net_create")
         self._h = h
 
     @staticmethod
-    def _make_desc(in_mean, in_scale, out_scale, gamma, shock_threshold, seed=0):
+    def _make_desc(in_mean, in_scale, out_scale, gamma, shock_threshold, seed=0, max_output=0.0):
         d = hf_net_desc()
         for r in range(4):
             d.in_mean[r] = float(in_mean[r])
@@ -319,18 +325,20 @@ class Net:
             d.out_scale[r] = float(out_scale[r])
         d.gamma = float(gamma)
         d.shock_threshold = float(shock_threshold)
+        d.max_output = float(max_output)
         d.seed = int(seed)
         return d
 
-    def set_desc(self, in_mean, in_scale, out_scale, gamma: float = 1.4, shock_threshold: float = 0.0) -> None:
-        self._desc = self._make_desc(in_mean, in_scale, out_scale, gamma, shock_threshold)
+    def set_desc(self, in_mean, in_scale, out_scale, gamma: float = 1.4, shock_threshold: float = 0.0,
+                 max_output: float = 0.0) -> None:
+        self._desc = self._make_desc(in_mean, in_scale, out_scale, gamma, shock_threshold, 0, max_output)
         _check(lib().hf_net_set_desc(self._h, C.byref(self._desc)), "hf_net_set_desc")
 
     @property
     def desc(self) -> dict:
         d = self._desc
         return {"in_mean": list(d.in_mean), "in_scale": list(d.in_scale), "out_scale": list(d.out_scale),
-                "gamma": d.gamma, "shock_threshold": d.shock_threshold}
+                "gamma": d.gamma, "shock_threshold": d.shock_threshold, "max_output": d.max_output}
 
     def close(self):
         if getattr(self, "_h", None):

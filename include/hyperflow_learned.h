@@ -11,9 +11,10 @@
  *     U <- U + out_scale * net((U - in_mean) * in_scale)
  *
  * cell by cell, except where the corrected density or pressure is not
- * positive or the shock sensor fires on the plain state (there the plain
- * step stands).  The sensor is the normalised second difference of the
- * pressure, |p+ - 2p + p-| / (p+ + 2p + p-), in x or y, above a threshold.
+ * positive, the shock sensor fires on the plain state, or the network output
+ * leaves the range it was trained on (there the plain step stands).  The
+ * sensor is the normalised second difference of the pressure,
+ * |p+ - 2p + p-| / (p+ + 2p + p-), in x or y, above a threshold.
  *
  * Device-pointer arguments (suffix _dev) are dense FP32 planes [4][ny][nx]
  * of (rho, rho*u, rho*v, E).  Network calls run on the stream set with
@@ -37,6 +38,8 @@ typedef struct {
     float out_scale[4];      /* correction = out_scale * network output */
     float gamma;             /* ideal-gas gamma for the pressure checks */
     float shock_threshold;   /* sensor threshold; <= 0 disables the sensor */
+    float max_output;        /* fall back where |network output| exceeds this; <= 0: no bound */
+    float reserved;
     uint64_t seed;           /* He-normal initial weights from this seed */
 } hf_net_desc;
 

@@ -157,10 +157,11 @@ def shock_sensor(U, gamma, theta, dtype=np.float64):
     return (sx > th) | (sy > th) | np.isnan(sx) | np.isnan(sy)
 
 
-def apply_correction(U, y, out_scale, gamma, theta, dtype=np.float64):
+def apply_correction(U, y, out_scale, gamma, theta, dtype=np.float64, max_output=0.0):
     """Corrected state and the fallback mask: the plain state stands where the
-    corrected density or pressure is not positive (or not finite) or the shock
-    sensor fires on the plain state.  Evaluated in dtype."""
+    corrected density or pressure is not positive (or not finite), the shock
+    sensor fires on the plain state, or some |y| exceeds max_output (when
+    positive).  Evaluated in dtype."""
     U = np.asarray(U, dtype=dtype)
     with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
         n = U + np.asarray(out_scale, dtype)[:, None, None] * np.asarray(y, dtype)
@@ -168,6 +169,8 @@ def apply_correction(U, y, out_scale, gamma, theta, dtype=np.float64):
     bad = ~(n[0] > 0) | ~(p > 0) | ~np.isfinite(p) | ~np.all(np.isfinite(n[1:]), axis=0)
     if theta > 0:
         bad |= shock_sensor(U, gamma, theta, dtype)
+    if max_output > 0:
+        bad |= np.any(np.abs(np.asarray(y, dtype)) > dtype(max_output), axis=0)
     out = np.where(bad[None], U, n)
     return out, bad
 

@@ -33,7 +33,7 @@ def test_library_exports_every_declared_entry_point():
 
 
 def test_net_desc_layout_and_param_count():
-    assert C.sizeof(native.hf_net_desc) == 14 * 4 + 8
+    assert C.sizeof(native.hf_net_desc) == 16 * 4 + 8
     assert native.Net.param_count() == 9 * 4 * 32 + 32 + 3 * (9 * 32 * 32 + 32) + 9 * 32 * 4 + 4
 
 

@@ -151,13 +151,14 @@ def test_fixed_dt_graph_equals_eager():
     np.testing.assert_array_equal(a.download(0), b.download(0))
 
 
-def test_correct_matches_cpu_apply_and_fallback():
+@pytest.mark.parametrize("max_output", [0.0, 0.5])
+def test_correct_matches_cpu_apply_and_fallback(max_output):
     """The in-place corrected state equals the CPU application of the same
     network output in FP32, cell by cell, including the fallback mask."""
     case, s, f = _fp32_solver(seed=6)
     g = case.ghost
     net = native.Net(in_mean=[1, 0, 0, 2.5], in_scale=[1, 1, 1, 1], out_scale=[0.5, 0.1, 0.1, 1.0], gamma=1.4,
-                     shock_threshold=0.2, seed=12)
+                     shock_threshold=0.2, seed=12, max_output=max_output)
     p = net.get_params()
     p[OFFS[4] + 9 * 32 * 4 + 0] = -0.3      # push the density down so some cells go non-positive
     net.set_params(p)
@@ -170,7 +171,7 @@ def test_correct_matches_cpu_apply_and_fallback():
     yv = y.cpu().numpy()
     nfb = s.correct(net, count=True)
     out = s.download(0)[:, g:-g, g:-g].astype(np.float32)
-    exp, bad = R.apply_correction(U, yv, net.desc["out_scale"], 1.4, 0.2, dtype=np.float32)
+    exp, bad = R.apply_correction(U, yv, net.desc["out_scale"], 1.4, 0.2, dtype=np.float32, max_output=max_output)
     say(f"learned: fallback {nfb} of {U[0].size} cells (cpu {int(bad.sum())})")
     assert nfb == int(bad.sum())
     assert 0 < nfb < U[0].size
@@ -187,3 +188,50 @@ def test_correct_all_fallback_leaves_state():
     nfb = s.correct(net, count=True)
     assert nfb == 128 * 64
     np.testing.assert_array_equal(s.download(0), before)
+
+
+def test_pairs_are_deterministic_and_restriction_conservative():
+    from hyperflow_amd import learned as L
+    say("learned: pair generation twice (seed 3)")
+    P1, T1, k1 = L.make_pairs([3], 4, 2, None, log=say)
+    P2, T2, k2 = L.make_pairs([3], 4, 2, None, log=say)
+    assert torch.equal(P1, P2) and torch.equal(T1, T2)
+    # the coarse run starts from the 2x2 block average of the fine state:
+    # cell sums agree to FP32 rounding
+    icf = L.fine_ic(3)
+    c = L.ic_on(L.COARSE, icf)[:, 2:-2, 2:-2]
+    np.testing.assert_allclose(4.0 * c.sum(axis=(1, 2)), icf.sum(axis=(1, 2)), rtol=1e-12)
+
+
+def test_trained_network_gain_over_plain_step():
+    """The committed network against the plain coarse step.  One correction
+    (K coarse steps from the restricted fine state of held-out seeds) must
+    remove a clear part of the density error; a 200-step rollout on unseen
+    seeds, corrected every 8 steps from step 24, must end closer to the
+    restricted 2048^2 solution than the plain 1024^2 run."""
+    from hyperflow_amd import learned as L
+    net = L.load()
+    meta = L.load_meta()
+    K = int(meta["trained_every"])
+    P, T, _ = L.make_pairs(meta["holdout_seeds"], K, 12, None, log=say)
+    y = torch.zeros_like(P[0])
+    sc = torch.tensor(net.desc["out_scale"], device="cuda")[:, None, None]
+    ratios = []
+    for i in range(P.shape[0]):
+        net.forward(P[i], y)
+        net.synchronize()
+        lp = float((P[i][0] - T[i][0]).abs().mean())
+        lc = float((P[i][0] + sc[0] * y[0] - T[i][0]).abs().mean())
+        ratios.append(lc / lp)
+    say(f"learned: one-correction density L1 ratio {np.mean(ratios):.3f} (per pair {np.round(ratios, 2).tolist()})")
+    assert np.mean(ratios) < 0.85
+    l1p, l1l = [], []
+    for seed in (5000, 5001, 5002):
+        icf = L.fine_ic(seed)
+        truth, _ = L.truth(icf, 200, seed)
+        rho_p, _ = L.run_plain(L.COARSE, icf, 200, seed)
+        rho_l, _, fb = L.run_learned(net, icf, 200, int(meta["every"]), seed, int(meta["start"]))
+        l1p.append(L.density_l1(rho_p, truth))
+        l1l.append(L.density_l1(rho_l, truth))
+        say(f"  seed {seed}: plain {l1p[-1]:.4e} learned {l1l[-1]:.4e} fallback {fb:.4f}")
+    assert np.mean(l1l) < np.mean(l1p)
