@@ -78,8 +78,10 @@ hipError_t launch_wgrad(int ci, int co, const WgradArgs& a, int* nwg, hipStream_
 int wgrad_workgroups(int nx, int ny, int rows);
 hipError_t launch_pack(const PackArgs& a, hipStream_t s);
 hipError_t launch_loss(const LossArgs& a, int* nblocks, hipStream_t s);
+// tmp: REDUCE_SPLIT * (9*ci*co + co) floats of scratch.
+constexpr int REDUCE_SPLIT = 16;
 hipError_t launch_reduce_grad(const float* part, int nparts, int ci, int co, int ci_r, int co_r, float* g,
-                              hipStream_t s);
+                              float* tmp, hipStream_t s);
 hipError_t launch_reduce_sum(const float* part, int n, float scale, float* out, hipStream_t s);
 hipError_t launch_adam(float* w, const float* g, float* m, float* v, int n, float lr, float b1, float b2, float eps,
                        float c1, float c2, hipStream_t s);

@@ -181,13 +181,15 @@ __global__ __launch_bounds__(256) void k_conv(const ConvArgs a)
 
 // ---------------------------------------------------------- weight gradient --
 // dW[tap][ci][co] = sum_p a[p + tap][ci] * gz[p][co],  db[co] = sum_p gz[p][co].
-// A workgroup owns a 64-pixel column strip over `rows` rows.  Per row it
-// stages gz (channel-major, gT[co][px]) and the three input rows shifted by
-// each tap (aT[tap][ci][px] = a[y+dy][x0+px+dx][ci]) in LDS, so each lane's
-// 8 consecutive pixels of one channel are one aligned 16-byte LDS read.  The
-// pixels are the MFMA reduction dimension: A = gz^T (rows = co), B = shifted
-// input (columns = ci).  Each wave owns a fixed set of (tap, co tile, ci
-// tile) products; partial sums go to part[workgroup].
+// A workgroup owns a 64-pixel column strip over `rows` rows.  Per output row
+// it stages gz (channel-major, gT[co][px]) and one new input row in LDS,
+// channel-major and shifted by each dx (aT[slot][dx][ci][px] =
+// a[row][x0+px+dx][ci]); the three input rows a tap can reach live in a ring
+// of three slots, so each input row is staged once.  Each lane's 8
+// consecutive pixels of one channel are then one aligned 16-byte LDS read.
+// The pixels are the MFMA reduction dimension: A = gz^T (rows = co), B =
+// shifted input (columns = ci).  Each wave owns a fixed set of (tap, co
+// tile, ci tile) products; partial sums go to part[workgroup].
 constexpr int LPX = 72;   // LDS row pitch in elements (64 pixels + 8 padding)
 
 template <int CI, int CO>
@@ -198,7 +200,7 @@ __global__ __launch_bounds__(256) void k_wgrad(const WgradArgs a)
     constexpr int MT = COL / 16, NT = CIL / 16;
     constexpr int ITEMS = 9 * MT * NT;
     constexpr int NPW = (ITEMS + 3) / 4;
-    __shared__ __attribute__((aligned(16))) uint16_t aT[9][CIL][LPX];
+    __shared__ __attribute__((aligned(16))) uint16_t aT[3][3][CIL][LPX];
     __shared__ __attribute__((aligned(16))) uint16_t gT[COL][LPX];
 
     const int tid = threadIdx.x;
@@ -214,16 +216,37 @@ __global__ __launch_bounds__(256) void k_wgrad(const WgradArgs a)
     if (CIL > CI)
         for (int i = tid; i < 9 * (CIL - CI) * LPX; i += 256) {
             const int t = i / ((CIL - CI) * LPX), rem = i % ((CIL - CI) * LPX);
-            aT[t][CI + rem / LPX][rem % LPX] = 0;
+            aT[t / 3][t % 3][CI + rem / LPX][rem % LPX] = 0;
         }
     if (COL > CO)
         for (int i = tid; i < (COL - CO) * LPX; i += 256) gT[CO + i / LPX][i % LPX] = 0;
+
+    // input row r (interior index, -1 <= r <= ny) into ring slot (r + 3) % 3
+    auto stage_row = [&](int r) {
+        const int slot = (r + 3) % 3;
+        for (int i = tid; i < 66 * (CI / 8); i += 256) {
+            const int ps = i / (CI / 8) - 1, cb = i % (CI / 8);
+            const uint4 v = *reinterpret_cast<const uint4*>(a.a + ((size_t)(r + 1) * W2 + x0 + ps + 1) * CI + cb * 8);
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int dxi = 0; dxi < 3; dxi++) {
+                const int px = ps - (dxi - 1);
+                if (px >= 0 && px < 64) {
+#pragma unroll
+                    for (int e = 0; e < 8; e++)
+                        aT[slot][dxi][cb * 8 + e][px] = (uint16_t)(w[e >> 1] >> (16 * (e & 1)));
+                }
+            }
+        }
+    };
 
     f32x4 acc[NPW];
 #pragma unroll
     for (int u = 0; u < NPW; u++) acc[u] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     float db = 0.0f;
 
+    stage_row(y0 - 1);
+    stage_row(y0);
     for (int y = y0; y < y1; y++) {
         __syncthreads();
         for (int i = tid; i < 64 * (CO / 8); i += 256) {
@@ -233,30 +256,18 @@ __global__ __launch_bounds__(256) void k_wgrad(const WgradArgs a)
 #pragma unroll
             for (int e = 0; e < 8; e++) gT[cb * 8 + e][px] = (uint16_t)(w[e >> 1] >> (16 * (e & 1)));
         }
-        for (int i = tid; i < 3 * 66 * (CI / 8); i += 256) {
-            const int dyi = i / (66 * (CI / 8)), rem = i % (66 * (CI / 8));
-            const int ps = rem / (CI / 8) - 1, cb = rem % (CI / 8);
-            const uint4 v = *reinterpret_cast<const uint4*>(a.a + ((size_t)(y + dyi) * W2 + x0 + ps + 1) * CI + cb * 8);
-            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-            for (int dxi = 0; dxi < 3; dxi++) {
-                const int px = ps - (dxi - 1);
-                if (px >= 0 && px < 64) {
-#pragma unroll
-                    for (int e = 0; e < 8; e++) aT[dyi * 3 + dxi][cb * 8 + e][px] = (uint16_t)(w[e >> 1] >> (16 * (e & 1)));
-                }
-            }
-        }
+        stage_row(y + 1);
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < NPW; u++) {
             const int q = wave + 4 * u;
             if (q < ITEMS) {
                 const int tap = q % 9, mt = (q / 9) % MT, nt = (q / 9) / MT;
+                const int slot = (y + tap / 3 + 2) % 3;   // input row y + dy, dy = tap/3 - 1
 #pragma unroll
                 for (int ks = 0; ks < 2; ks++) {
                     const bf16x8 A = load8(&gT[16 * mt + j][32 * ks + 8 * g]);
-                    const bf16x8 B = load8(&aT[tap][16 * nt + j][32 * ks + 8 * g]);
+                    const bf16x8 B = load8(&aT[slot][tap % 3][16 * nt + j][32 * ks + 8 * g]);
                     acc[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[u], 0, 0, 0);
                 }
             }
@@ -349,35 +360,45 @@ __global__ __launch_bounds__(256) void k_loss(const LossArgs a)
     if (threadIdx.x == 0) a.part[blockIdx.x] = red[0];
 }
 
-// Sum of the workgroup partials: 64 consecutive partial entries per block,
-// four interleaved partial subsets per entry combined in a fixed order, so
-// the result is deterministic.  Output in the layer's real layout
-// W[9][ci_r][co_r] followed by b[co_r] (padding channels dropped).
-__global__ __launch_bounds__(256) void k_reduce_grad(const float* __restrict__ part, int nparts, int CI, int CO,
-                                                     int ci_r, int co_r, float* __restrict__ gout)
+// Sum of the workgroup partials in two deterministic stages.  Stage 1: block
+// (bx, by) adds partials [by*chunk, (by+1)*chunk) of 64 consecutive entries,
+// four interleaved subsets per entry combined in a fixed order, into
+// tmp[by].  Stage 2 adds the stage-1 rows in order and writes the layer's
+// real layout W[9][ci_r][co_r] followed by b[co_r] (padding channels
+// dropped).
+__global__ __launch_bounds__(256) void k_reduce_stage1(const float* __restrict__ part, int nparts, int stride,
+                                                       int chunk, float* __restrict__ tmp)
 {
     __shared__ float red[4][64];
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    const int stride = 9 * CI * CO + CO;
     const int s = blockIdx.x * 64 + tx;
+    const int w0 = blockIdx.y * chunk, w1 = min(nparts, w0 + chunk);
     float acc = 0.0f;
     if (s < stride)
-        for (int w = ty; w < nparts; w += 4) acc += part[(size_t)w * stride + s];
+        for (int w = w0 + ty; w < w1; w += 4) acc += part[(size_t)w * stride + s];
     red[ty][tx] = acc;
     __syncthreads();
-    if (ty == 0 && s < stride) {
-        const float v = (red[0][tx] + red[1][tx]) + (red[2][tx] + red[3][tx]);
-        int dst = -1;
-        if (s < 9 * CI * CO) {
-            const int tap = s / (CI * CO), rem = s % (CI * CO);
-            const int ci = rem / CO, co = rem % CO;
-            if (ci < ci_r && co < co_r) dst = (tap * ci_r + ci) * co_r + co;
-        } else {
-            const int co = s - 9 * CI * CO;
-            if (co < co_r) dst = 9 * ci_r * co_r + co;
-        }
-        if (dst >= 0) gout[dst] = v;
+    if (ty == 0 && s < stride) tmp[(size_t)blockIdx.y * stride + s] = (red[0][tx] + red[1][tx]) + (red[2][tx] + red[3][tx]);
+}
+
+__global__ __launch_bounds__(256) void k_reduce_grad(const float* __restrict__ part, int nparts, int CI, int CO,
+                                                     int ci_r, int co_r, float* __restrict__ gout)
+{
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    const int stride = 9 * CI * CO + CO;
+    if (s >= stride) return;
+    float v = 0.0f;
+    for (int w = 0; w < nparts; w++) v += part[(size_t)w * stride + s];
+    int dst = -1;
+    if (s < 9 * CI * CO) {
+        const int tap = s / (CI * CO), rem = s % (CI * CO);
+        const int ci = rem / CO, co = rem % CO;
+        if (ci < ci_r && co < co_r) dst = (tap * ci_r + ci) * co_r + co;
+    } else {
+        const int co = s - 9 * CI * CO;
+        if (co < co_r) dst = 9 * ci_r * co_r + co;
     }
+    if (dst >= 0) gout[dst] = v;
 }
 
 __global__ __launch_bounds__(256) void k_reduce_sum(const float* __restrict__ part, int n, float scale,
@@ -514,10 +535,14 @@ hipError_t launch_loss(const LossArgs& a, int* nblocks, hipStream_t s)
 }
 
 hipError_t launch_reduce_grad(const float* part, int nparts, int ci, int co, int ci_r, int co_r, float* g,
-                              hipStream_t s)
+                              float* tmp, hipStream_t s)
 {
-    const int n = 9 * ci * co + co;
-    hipLaunchKernelGGL(k_reduce_grad, dim3((n + 63) / 64), dim3(256), 0, s, part, nparts, ci, co, ci_r, co_r, g);
+    const int stride = 9 * ci * co + co;
+    const int chunk = (nparts + REDUCE_SPLIT - 1) / REDUCE_SPLIT;
+    hipLaunchKernelGGL(k_reduce_stage1, dim3((stride + 63) / 64, REDUCE_SPLIT), dim3(256), 0, s, part, nparts, stride,
+                       chunk, tmp);
+    hipLaunchKernelGGL(k_reduce_grad, dim3((stride + 255) / 256), dim3(256), 0, s, tmp, REDUCE_SPLIT, ci, co, ci_r,
+                       co_r, g);
     return hipGetLastError();
 }
 

@@ -64,6 +64,7 @@ struct hf_net {
     float* y = nullptr;
     uint8_t* shock = nullptr;
     float* part = nullptr;
+    float* part2 = nullptr;   // second stage of the gradient reduction
     float* lossp = nullptr;
     float* loss = nullptr;
     uint32_t* fallback = nullptr;
@@ -83,9 +84,10 @@ void free_images(hf_net* n)
     hipFree(n->y);
     hipFree(n->shock);
     hipFree(n->part);
+    hipFree(n->part2);
     hipFree(n->lossp);
     n->x = n->g8 = nullptr;
-    n->y = n->part = n->lossp = nullptr;
+    n->y = n->part = n->part2 = n->lossp = nullptr;
     n->shock = nullptr;
     n->nx = n->ny = 0;
 }
@@ -114,6 +116,7 @@ int ensure_images(hf_net* n, int nx, int ny)
     HF_HIP(hipMalloc(&n->shock, (size_t)nx * ny));
     const int nwg = hfn::wgrad_workgroups(nx, ny, WGRAD_ROWS);
     HF_HIP(hipMalloc(&n->part, (size_t)nwg * (9 * 32 * 32 + 32) * sizeof(float)));
+    HF_HIP(hipMalloc(&n->part2, (size_t)hfn::REDUCE_SPLIT * (9 * 32 * 32 + 32) * sizeof(float)));
     const int nloss = (int)(((int64_t)nx * ny + 255) / 256);
     HF_HIP(hipMalloc(&n->lossp, (size_t)nloss * sizeof(float)));
     n->nx = nx;
@@ -199,7 +202,7 @@ int wgrad(hf_net* n, int l, const uint16_t* a_in, const uint16_t* gz, int nx, in
     w.rows = WGRAD_ROWS;
     int nwg = 0;
     HF_HIP(hfn::launch_wgrad(L.cis, L.cos, w, &nwg, s));
-    HF_HIP(hfn::launch_reduce_grad(n->part, nwg, L.cis, L.cos, L.ci, L.co, n->grads + L.off, s));
+    HF_HIP(hfn::launch_reduce_grad(n->part, nwg, L.cis, L.cos, L.ci, L.co, n->grads + L.off, n->part2, s));
     return HF_OK;
 }
 

@@ -413,6 +413,34 @@ def summarize(rows, resolutions, every):
     return out
 
 
+def bench_net(n: int = 1024, reps: int = 50) -> dict:
+    """Average time of the forward pass and of a training step on an n x n
+    state (random weights, synthetic smooth state)."""
+    net = native.Net(in_mean=[1, 0, 0, 2.5], in_scale=[1, 1, 1, 1], out_scale=[0.01] * 4, seed=1)
+    yy, xx = torch.meshgrid(torch.linspace(0, 1, n, device="cuda"), torch.linspace(0, 1, n, device="cuda"),
+                            indexing="ij")
+    u = torch.stack([1 + 0.2 * torch.sin(6 * xx), 0.1 * torch.cos(4 * yy), 0.1 * torch.sin(5 * xx),
+                     2.5 + 0.1 * torch.cos(3 * (xx + yy))]).contiguous()
+    t = (u * 1.001).contiguous()
+    y = torch.zeros_like(u)
+    for _ in range(3):
+        net.forward(u, y)
+        net.train_step(u, t, 1e-6)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        net.forward(u, y)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        net.train_step(u, t, 1e-6)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    flops = 2 * 9 * sum(ci * co for ci, co in ((4, 32), (32, 32), (32, 32), (32, 32), (32, 4))) * n * n
+    return {"n": n, "forward_ms": (t1 - t0) / reps * 1e3, "train_step_ms": (t2 - t1) / reps * 1e3,
+            "forward_tflops": flops / ((t1 - t0) / reps) / 1e12}
+
+
 # ------------------------------------------------------------------ main --
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m hyperflow_amd.learned")
@@ -431,6 +459,9 @@ def main(argv=None):
     t.add_argument("--holdout-seeds", type=int, default=2)
     t.add_argument("--out", default=WEIGHTS)
     t.add_argument("--save-rounds", default=None, help="directory for the weights after every round")
+    b = sub.add_parser("bench", help="time the network's forward pass and training step")
+    b.add_argument("--n", type=int, default=1024)
+    b.add_argument("--reps", type=int, default=50)
     e = sub.add_parser("evaluate")
     e.add_argument("--weights", default=WEIGHTS)
     e.add_argument("--seeds", type=int, nargs="+", default=[5000, 5001, 5002])
@@ -445,6 +476,9 @@ def main(argv=None):
     def log(msg):
         print(msg, flush=True)
 
+    if args.cmd == "bench":
+        log(json.dumps(bench_net(args.n, args.reps)))
+        return 0
     if args.cmd == "train":
         ensure_tuned((COARSE, FINE), log=log)
         seeds = list(range(args.seeds))
