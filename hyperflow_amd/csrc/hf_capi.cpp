@@ -566,7 +566,7 @@ int hf_create(const hf_case_desc* desc, hf_ctx** out)
     // Items are taken from a queue in list order, so the list ends with small
     // items: the last round of a launch then finishes together instead of
     // waiting on a few long items.  Main launch: `rows`-row items over the
-    // first 3/4 of each block's rows, then 16-row items.  A fix-up launch
+    // first 3/4 of each block's rows, then 32-row items (measured among 4..48).  A fix-up launch
     // computes a single block on the whole GPU: 24-row items, then 12-row
     // items (measured fastest on the 8-block channel among 8..32 rows).
     auto add_rows = [&](int b, int y0, int y1, int r) {
@@ -576,7 +576,7 @@ int hf_create(const hf_case_desc* desc, hf_ctx** out)
             for (int s = 0; s < strips; s++) c->items.push_back(HfItem{b, s * W, c->g + y, std::min(r, y1 - y)});
     };
     auto split = [&](int ny, int r) { return std::max(0, std::min(ny, (ny * 3 / 4) / r * r)); };
-    const int tail_rows = std::max(1, std::min(rows, 16));
+    const int tail_rows = std::max(1, std::min(rows, 32));
     for (int b = 0; b < c->nb; b++) {
         c->item_begin[b] = (int)c->items.size();
         add_rows(b, 0, c->blocks[b].ny, rows);          // contiguous per block (class-level launches)
