@@ -90,3 +90,49 @@ def tune(cases, fields: Sequence[np.ndarray], precision: str = "fp64", dt_mode: 
         with open(CACHE, "w") as f:
             json.dump(cache, f, indent=1, sort_keys=True)
     return result
+
+
+def _workloads(names):
+    """The benchmark workloads by name: (label, cases, fields, precision, dt_mode)."""
+    from . import case as C
+    out = []
+    for name in names:
+        if name == "c1":
+            c = C.config1(0)
+            out.append(("c1", [c], [c.initial_fields()[0]], "fp64", "compat"))
+        elif name == "c2":
+            c = C.config2(1024)
+            out.append(("c2", [c], c.initial_fields(), "fp64", "compat"))
+        elif name in ("c3", "c3_fp32"):
+            c = C.config3(0, 8192)
+            f = C.perturbed_fields(c, 0)
+            for prec in (("fp64", "fp32") if name == "c3" else ("fp32",)):
+                out.append((f"c3_{prec}", [c], f, prec, "compat"))
+        elif name == "c4":
+            cases = [C.quad_case(nx, ny, 1000 + k, scheme="MUSCLHancock", limiter="VanLeer", cfl=0.5)
+                     for k, (nx, ny) in enumerate(C.batched_sizes(512, 0))]
+            out.append(("c4", cases, [c.initial_fields()[0] for c in cases], "fp32", "compat"))
+        elif name == "c5":
+            for n in (1024, 1280, 1536, 1792, 2048):
+                c = C.config3(0, n)
+                out.append((f"c5_{n}", [c], C.perturbed_fields(c, 0), "fp32", "uniform"))
+        else:
+            raise SystemExit(f"unknown workload {name}")
+    return out
+
+
+def main(argv=None) -> int:
+    import argparse
+    ap = argparse.ArgumentParser(prog="python -m hyperflow_amd.autotune",
+                                 description="tune tile rows of the benchmark workloads into the cache")
+    ap.add_argument("workloads", nargs="*", default=["c1", "c2", "c3", "c4", "c5"])
+    args = ap.parse_args(argv)
+    for label, cases, fields, prec, mode in _workloads(args.workloads):
+        r = tune(cases, fields, precision=prec, dt_mode=mode)
+        print(f"{label}: tile_rows {r['tile_rows']} ({r['ms_per_step']:.4f} ms/step) {r['timings']}", flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    import sys
+    sys.exit(main())
