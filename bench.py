@@ -19,6 +19,9 @@ Other workloads of the same path (not the headline line):
                     solution (density L1, wall time, speed-up at equal L1)
     --workload copy a 1 GiB device copy: measured copy bandwidth, and the
                     calibration run for the HBM traffic counters
+    --workload u1   unstructured Godunov (fvm_2d_uns), FP64: the shipped
+                    scramjet case on its coarse block mesh refined --refine
+                    times per direction (default 8: 1 469 440 cells)
 """
 from __future__ import annotations
 
@@ -125,6 +128,70 @@ def traffic_from_profiles(workload: str):
         return None
 
 
+UNS_BYTES_PER_CELL = 64 + 192   # state in + out; neighbours, lengths, 16 rotation terms, 1/V, shortest edge
+
+
+def uns_line(steps: int, warmup: int, refine: int, with_cpu: bool):
+    """The unstructured solver on the shipped scramjet case: its coarse block
+    mesh (the reference's scramjet_kumar_coarse_mesh.json, kept with the
+    scramjet golden) with every block's cell counts multiplied by `refine`,
+    Constant inflow initial state, device-resident steps (one launch each)."""
+    import copy
+    from hyperflow_amd import uns as U
+    with open(os.path.join(ROOT, "tests", "golden", "uns_kumar_coarse.json")) as f:
+        spec = json.load(f)
+    blocks = copy.deepcopy(spec["mesh_blocks"])
+    for b in blocks:
+        b["cells"] = [b["cells"][0] * refine, b["cells"][1] * refine]
+    q, c = U.block_mesh(blocks)
+    case = spec["case"]
+    gamma, cfl = case["model"]["gamma"], case["simulation"]["cfl"]
+    s = U.UnsSolver(q, c, U.prim_to_cons(case["mesh"]["boundaries"]["inlet"], gamma), gamma, cfl)
+    s.upload(U.initial_fields(q, case["initial_condition"], gamma))
+    s.run(warmup)
+    s.synchronize()
+    s.profile(True)
+    s.kernel_stats()
+    t0 = time.perf_counter()
+    s.run(steps)
+    s.synchronize()
+    wall = time.perf_counter() - t0
+    launches, kernel_ms = s.kernel_stats()
+    s.close()
+    n = len(q)
+    avg_ms = kernel_ms / launches
+    achieved = n * UNS_BYTES_PER_CELL / (avg_ms * 1e-3) / 1e9
+    line = {
+        "metric": "unstructured Godunov-HLLC Mcell-updates/s", "value": round(n * steps / wall / 1e6, 2),
+        "unit": "Mcell-updates/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": round(wall / steps * 1e3, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (the case's constant inflow state)",
+        "config": {"workload": f"scramjet_kumar_godunov_hllc on the coarse block mesh refined {refine}x per "
+                               f"direction ({n} quadrilateral cells), Godunov + rotated HLLC, forward Euler",
+                   "cells": n, "cfl": cfl},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(f"u1_r{refine}"),
+                     "kernel": "k_uns_step", "avg_launch_ms": round(avg_ms, 5), "launches": launches,
+                     "algorithmic_bytes_per_launch": n * UNS_BYTES_PER_CELL, "bytes_per_cell": UNS_BYTES_PER_CELL},
+    }
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import hfo
+        q0, c0 = U.block_mesh(spec["mesh_blocks"])
+        o = hfo.UnsOracle(q0, c0, U.prim_to_cons(case["mesh"]["boundaries"]["inlet"], gamma), gamma, cfl)
+        o.set_fields(U.initial_fields(q0, case["initial_condition"], gamma))
+        k = 400
+        t = time.perf_counter()
+        o.run(k)
+        el = time.perf_counter() - t
+        line["cpu_baseline"] = {"value": round(len(q0) * k / el / 1e6, 3), "unit": "Mcell-updates/s", "cores": 1,
+                                "kind": "port",
+                                "sample": f"the same case on the unrefined coarse mesh ({len(q0)} cells), {k} steps, "
+                                          "CPU oracle (C, serial); the reference itself ran 0.075 Mcell-updates/s "
+                                          "on this mesh when its goldens were written"}
+    return line
+
+
 def build_workload(name: str, n: int, precision: str, seed: int = 0):
     from hyperflow_amd import case as C
     if name == "c1":
@@ -178,7 +245,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "c5", "copy"], default="c2")
+    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "c5", "copy", "u1"], default="c2")
+    ap.add_argument("--refine", type=int, default=8, help="u1: cells per coarse cell and direction")
     ap.add_argument("--n", type=int, default=1024, help="cells per block side (c2), grid side (c3)")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default=None)
     ap.add_argument("--tile-rows", type=int, default=None, help="default: autotuned value if cached")
@@ -196,6 +264,9 @@ def main():
         return
     if args.workload == "c5":
         print(json.dumps(learned_line()), flush=True)
+        return
+    if args.workload == "u1":
+        print(json.dumps(uns_line(args.steps, args.warmup, args.refine, not args.no_cpu_baseline)), flush=True)
         return
 
     from hyperflow_amd import autotune, native

@@ -32,10 +32,13 @@ UNITS = [
     ("hf_cnn.hip", DEVICE + ["-ffp-contract=off"], True),
     ("hf_capi.cpp", ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-O2", "-ffp-contract=off"], True),
     ("hf_learned.cpp", ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-O2"], True),
+    ("hf_uns.hip", DEVICE + ["-ffp-contract=off"], True),
+    ("hf_uns.cpp", ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-O2", "-ffp-contract=off"], True),
     ("host/hf_json.cpp", ["-x", "c++", "-O2"], False),
     ("host/hf_host.cpp", ["-x", "c++", "-O2", "-ffp-contract=off"], False),
 ]
 CLI = os.path.join(HERE, "bin", "hyperflow")
+CLIS = ((CLI, "host/hf_cli.cpp"), (os.path.join(HERE, "bin", "hyperflow_uns"), "host/hf_uns_cli.cpp"))
 
 
 def _compile(unit, flags, hidden, outdir):
@@ -60,16 +63,17 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout)
     os.replace(tmp, LIB)
-    # the `hyperflow` command-line driver, linked against the library
+    # the `hyperflow` and `hyperflow_uns` command-line drivers, linked against the library
     os.makedirs(os.path.dirname(CLI), exist_ok=True)
-    cli_obj = _compile("host/hf_cli.cpp", ["-x", "c++", "-O2"], False, outdir)
-    cmd = [HIPCC, "-o", CLI + ".tmp", cli_obj, f"-L{HERE}", "-lhyperflow_amd", "-Wl,-rpath,$ORIGIN/.."]
-    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("CLI link failed:\n" + r.stdout)
-    os.replace(CLI + ".tmp", CLI)
+    for exe, src in CLIS:
+        cli_obj = _compile(src, ["-x", "c++", "-O2"], False, outdir)
+        cmd = [HIPCC, "-o", exe + ".tmp", cli_obj, f"-L{HERE}", "-lhyperflow_amd", "-Wl,-rpath,$ORIGIN/.."]
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"{src}: link failed:\n" + r.stdout)
+        os.replace(exe + ".tmp", exe)
     if verbose:
-        print(f"built {LIB} and {CLI}")
+        print(f"built {LIB} and {', '.join(e for e, _ in CLIS)}")
     return LIB
 
 

@@ -1,0 +1,159 @@
+// hf_uns.hip -- gfx950 kernels of the unstructured quadrilateral-cell
+// Godunov solver (src/fvm_2d_uns), FP64, bit-exact with the reference:
+// compiled with -ffp-contract=off, every expression in the reference's order.
+#include <hip/hip_runtime.h>
+
+#include "hf_common.h"
+#include "hf_physics.h"
+#include "hf_uns.h"
+
+namespace hfu {
+
+using hf::MathF64;
+typedef double R;
+
+__device__ __forceinline__ unsigned long long bits_of(double d) { return (unsigned long long)__double_as_longlong(d); }
+__device__ __forceinline__ double double_of(unsigned long long b) { return __longlong_as_double((long long)b); }
+
+// EulerIdealGasModel::rotate_flow_values (model/euler_ideal_gas/euler_ideal_gas.cpp:152-164)
+__device__ __forceinline__ void rotate(double c, double s, const double W[4], double Rv[4])
+{
+    Rv[0] = W[0];
+    Rv[1] = c * W[1] + s * W[2];
+    Rv[2] = -1.0 * s * W[1] + c * W[2];
+    Rv[3] = W[3];
+}
+
+// TimeStep::operator()'s per-cell quotient (scheme/timestep/timestep.cpp:41-52)
+__device__ __forceinline__ double cell_dt(const double U[4], double dr, double gamma, double g1)
+{
+    double u, v, p, E;
+    hf::prims<MathF64>(U[0], U[1], U[2], U[3], g1, u, v, p, E);
+    const double a = hf::sound<MathF64>(gamma, p, U[0]);
+    return dr / (__builtin_fabs(__builtin_sqrt(u * u + v * v)) + a);
+}
+
+// minimum over the wave of bit patterns (positive doubles order like their bits;
+// NaN patterns are larger than every number, as std::min ignores NaN quotients)
+__device__ __forceinline__ unsigned long long wave_min_bits(unsigned long long b)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(b, off);
+        b = o < b ? o : b;
+    }
+    return b;
+}
+
+// GodunovScheme::operator() (scheme/godunov/godunov.cpp:31-77), boundary
+// states of BoundaryConditions (scheme/boundary/boundary.cpp:29-69), and
+// ForwardEulerODESolver (scheme/ode/forward_euler/forward_euler.cpp:16-30).
+__global__ __launch_bounds__(256) void k_uns_step(const UnsMesh m, const UnsStep s)
+{
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    const int n = m.n;
+    const double gamma = m.gamma, g1 = m.gamma - 1.0;
+    double dt;
+    if (s.use_fixed) {
+        dt = s.fixed_dt;
+    } else {
+        const double cf = (s.kcall <= 10) ? 0.2 * m.cfl : m.cfl;   // timestep.cpp:57-61
+        dt = cf * double_of(s.slots[s.rd]);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (!s.use_fixed && s.kcall >= 1 && s.kcall <= s.hist_cap) s.hist[s.kcall - 1] = dt;
+        if (s.rs >= 0) s.slots[s.rs] = bits_of(1e16);
+    }
+    unsigned long long cand = ~0ull;
+    if (c < n) {
+        double U[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) U[k] = s.A[(size_t)k * n + c];
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int nb = m.nbr[(size_t)e * n + c];
+            const double cr = m.cs[(size_t)(4 * e + 0) * n + c], sr = m.cs[(size_t)(4 * e + 1) * n + c];
+            const double cb = m.cs[(size_t)(4 * e + 2) * n + c], sb = m.cs[(size_t)(4 * e + 3) * n + c];
+            double V[4];
+            if (nb >= 0) {
+#pragma unroll
+                for (int k = 0; k < 4; k++) V[k] = s.A[(size_t)k * n + nb];
+            } else if (nb == -1) {   // reflective (boundary.cpp:48-56)
+                double t[4];
+                rotate(cr, sr, U, t);
+                t[1] *= -1.0;
+                rotate(cb, sb, t, V);
+            } else if (nb == -2) {   // transmissive
+#pragma unroll
+                for (int k = 0; k < 4; k++) V[k] = U[k];
+            } else {                 // inlet
+#pragma unroll
+                for (int k = 0; k < 4; k++) V[k] = m.inlet[k];
+            }
+            double rU[4], rV[4], F[4], f[4];
+            rotate(cr, sr, U, rU);
+            rotate(cr, sr, V, rV);
+            const hf::State<R> L = hf::make_state<MathF64>(rU, gamma, g1);
+            const hf::State<R> Rs = hf::make_state<MathF64>(rV, gamma, g1);
+            hf::hllc<MathF64, true>(L, Rs, F);
+            rotate(cb, sb, F, f);
+            const double len = m.len[(size_t)e * n + c];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                f[k] *= len;
+                acc[k] += f[k];
+            }
+        }
+        const double iv = m.inv_vol[c];
+        double N[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            double x = acc[k];
+            x *= iv;
+            x *= -1.0;
+            x *= dt;
+            x += U[k];
+            N[k] = x;
+            s.B[(size_t)k * n + c] = x;
+        }
+        if (s.wr >= 0) cand = bits_of(cell_dt(N, m.dr[c], gamma, g1));
+    }
+    if (s.wr >= 0) {
+        cand = wave_min_bits(cand);
+        atomicMin(&s.slots[s.wr], cand);   // every lane holds the wave minimum
+    }
+}
+
+__global__ __launch_bounds__(256) void k_uns_scan(const UnsMesh m, const double* __restrict__ A,
+                                                  unsigned long long* slot)
+{
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    const int n = m.n;
+    unsigned long long cand = ~0ull;
+    if (c < n) {
+        double U[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) U[k] = A[(size_t)k * n + c];
+        cand = bits_of(cell_dt(U, m.dr[c], m.gamma, m.gamma - 1.0));
+    }
+    cand = wave_min_bits(cand);
+    atomicMin(slot, cand);
+}
+
+__global__ void k_uns_reset(unsigned long long* slot) { *slot = bits_of(1e16); }
+
+hipError_t launch_uns_step(const UnsMesh& m, const UnsStep& s, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_uns_step, dim3((m.n + 255) / 256), dim3(256), 0, st, m, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_uns_scan(const UnsMesh& m, const double* A, unsigned long long* slots, int slot, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_uns_reset, dim3(1), dim3(1), 0, st, slots + slot);
+    hipLaunchKernelGGL(k_uns_scan, dim3((m.n + 255) / 256), dim3(256), 0, st, m, A, slots + slot);
+    return hipGetLastError();
+}
+
+}  // namespace hfu

@@ -9,11 +9,16 @@
  * reference file:line it restates; paths are relative to the reference
  * checkout, under src/fvm_2d_cart/ unless stated otherwise.
  */
+#define _GNU_SOURCE   /* sincos */
 #include "hf_oracle.h"
 
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846   /* the value of glibc's M_PI */
+#endif
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -579,4 +584,212 @@ int hfo_run(hfo_mesh* m, int nsteps, double* dt_steps, double* dt_calls, int max
         memcpy(dt_calls, m->log + first, sizeof(double) * k);
     }
     return made;
+}
+
+
+/* ===================================================================== */
+/* Unstructured quadrilateral-cell Godunov solver (src/fvm_2d_uns).       */
+/* ===================================================================== */
+
+struct hfo_uns {
+    int n;
+    double gamma, cfl, inlet[4];
+    int* nbr;          /* [n][4] neighbour cell or boundary code */
+    double* len;       /* [n][4] edge length */
+    double* cs;        /* [n][4][4] cos, sin of the outward normal angle, then of its negative */
+    double* inv_vol;   /* [n] */
+    double* dr;        /* [n] incircle radius (shortest edge) */
+    double* cen;       /* [n][2] */
+    double* vol;       /* [n] */
+    double* U;         /* [n][4] */
+    double* L;         /* [n][4] scratch */
+    unsigned steps;
+};
+
+/* Edge(start, end) members (geometry/edge/edge.cpp:12-238, tol 1e-12). */
+static void uns_edge(const double* a, const double* b, double* len, double* cs)
+{
+    const double px = b[0] - a[0], py = b[1] - a[1];           /* calculate_point_vector */
+    *len = sqrt((0.0 + px * px) + py * py);                     /* l2_norm = sqrt(dot) */
+    double n0 = py, n1 = -1.0 * px;                             /* calculate_outward_unit_normal */
+    const double nl = sqrt(n0 * n0 + n1 * n1);
+    n0 /= nl;
+    n1 /= nl;
+    const double deg = atan2(n1, n0) * 180.0 / M_PI;            /* calculate_outward_normal_angle */
+    const double r = deg * (M_PI / 180.0);                      /* rotate_flow_values (euler_ideal_gas.cpp:155) */
+    const double rb = (-1.0 * deg) * (M_PI / 180.0);            /* the rotation back (godunov.cpp:62) */
+    /* rotate_flow_values takes cos and sin of one angle; the reference's
+     * build (g++ -O3, build/fvm_2d_uns/makefile) merges the pair into one
+     * glibc sincos call, whose results differ from separate cos and sin in
+     * the last bit for some angles.  Called explicitly here. */
+    sincos(r, &cs[1], &cs[0]);
+    sincos(rb, &cs[3], &cs[2]);
+}
+
+static int uns_opposite(const double* a0, const double* a1, const double* b0, const double* b1)
+{
+    /* Edge::opposite_edges_equal (edge.cpp:47-66): rhs start == end, rhs end == start */
+    const double tol = 1e-12;
+    return fabs(b0[0] - a1[0]) < tol && fabs(b0[1] - a1[1]) < tol && fabs(b1[0] - a0[0]) < tol &&
+           fabs(b1[1] - a0[1]) < tol;
+}
+
+hfo_uns* hfo_uns_create(int n, const double* V, const int* codes, double gamma, double cfl, const double* inlet)
+{
+    hfo_uns* m = (hfo_uns*)calloc(1, sizeof(hfo_uns));
+    m->n = n;
+    m->gamma = gamma;
+    m->cfl = cfl;
+    for (int k = 0; k < 4; k++) m->inlet[k] = inlet[k];
+    m->nbr = (int*)malloc(sizeof(int) * 4 * n);
+    m->len = (double*)malloc(sizeof(double) * 4 * n);
+    m->cs = (double*)malloc(sizeof(double) * 16 * n);
+    m->inv_vol = (double*)malloc(sizeof(double) * n);
+    m->dr = (double*)malloc(sizeof(double) * n);
+    m->cen = (double*)malloc(sizeof(double) * 2 * n);
+    m->vol = (double*)malloc(sizeof(double) * n);
+    m->U = (double*)calloc(4 * (size_t)n, sizeof(double));
+    m->L = (double*)calloc(4 * (size_t)n, sizeof(double));
+    m->steps = 1;
+    for (int c = 0; c < n; c++) {
+        const double* q = V + 8 * c;
+        /* Polygon::calculate_volume / calculate_centroid (polygon.cpp:162-210) */
+        double vol = 0.0;
+        for (int i = 0; i < 4; i++) {
+            const int j = (i + 1) % 4;
+            vol += (q[2 * i] * q[2 * j + 1] - q[2 * j] * q[2 * i + 1]);
+        }
+        vol *= 0.5;
+        double cx = 0.0, cy = 0.0;
+        for (int i = 0; i < 4; i++) {
+            const int j = (i + 1) % 4;
+            const double second = (q[2 * i] * q[2 * j + 1] - q[2 * j] * q[2 * i + 1]);
+            cx += ((q[2 * i] + q[2 * j]) * second);
+            cy += ((q[2 * i + 1] + q[2 * j + 1]) * second);
+        }
+        m->vol[c] = vol;
+        m->cen[2 * c] = cx / (6.0 * vol);
+        m->cen[2 * c + 1] = cy / (6.0 * vol);
+        m->inv_vol[c] = 1.0 / vol;                               /* godunov.cpp:71 */
+        double shortest = 1e16;                                  /* Polygon::radius_incircle, 4 sides */
+        for (int e = 0; e < 4; e++) {
+            uns_edge(q + 2 * e, q + 2 * ((e + 1) % 4), &m->len[4 * c + e], &m->cs[16 * c + 4 * e]);
+            shortest = (shortest < m->len[4 * c + e]) ? shortest : m->len[4 * c + e];   /* std::min(len, shortest) */
+        }
+        m->dr[c] = shortest;
+    }
+    /* LoadStructuredMesh::load_structured_mesh (mesh_structured.cpp:93-115):
+     * the edge's boundary code, replaced by the last cell (in cell, edge
+     * order) that has the same edge in the opposite direction */
+    for (int c = 0; c < n; c++)
+        for (int e = 0; e < 4; e++) {
+            const double* a0 = V + 8 * c + 2 * e;
+            const double* a1 = V + 8 * c + 2 * ((e + 1) % 4);
+            int nb = codes[4 * c + e];
+            for (int d = 0; d < n; d++)
+                for (int f = 0; f < 4; f++)
+                    if (uns_opposite(a0, a1, V + 8 * d + 2 * f, V + 8 * d + 2 * ((f + 1) % 4))) nb = d;
+            m->nbr[4 * c + e] = nb;
+        }
+    return m;
+}
+
+void hfo_uns_destroy(hfo_uns* m)
+{
+    if (!m) return;
+    free(m->nbr);
+    free(m->len);
+    free(m->cs);
+    free(m->inv_vol);
+    free(m->dr);
+    free(m->cen);
+    free(m->vol);
+    free(m->U);
+    free(m->L);
+    free(m);
+}
+
+void hfo_uns_connectivity(const hfo_uns* m, int* out) { memcpy(out, m->nbr, sizeof(int) * 4 * m->n); }
+
+void hfo_uns_geometry(const hfo_uns* m, double* centroids, double* volumes)
+{
+    memcpy(centroids, m->cen, sizeof(double) * 2 * m->n);
+    memcpy(volumes, m->vol, sizeof(double) * m->n);
+}
+
+void hfo_uns_set_fields(hfo_uns* m, const double* U) { memcpy(m->U, U, sizeof(double) * 4 * m->n); }
+void hfo_uns_get_fields(const hfo_uns* m, double* U) { memcpy(U, m->U, sizeof(double) * 4 * m->n); }
+
+/* TimeStep::operator() (scheme/timestep/timestep.cpp:29-62) */
+double hfo_uns_time_step(hfo_uns* m)
+{
+    double dtr = 1e16;
+    for (int c = 0; c < m->n; c++) {
+        double W[4];
+        hfo_cons_to_prim(m->gamma, m->U + 4 * c, W);
+        const double u = W[1], v = W[2];
+        const double a = sqrt(m->gamma * W[3] / W[0]);
+        dtr = smin(dtr, m->dr[c] / (fabs(sqrt(u * u + v * v)) + a));
+    }
+    const double cfl = (m->steps <= 10) ? 0.2 * m->cfl : m->cfl;
+    m->steps++;
+    return cfl * dtr;
+}
+
+/* EulerIdealGasModel::rotate_flow_values (euler_ideal_gas.cpp:152-164) */
+static void uns_rotate(double c, double s, const double* W, double* R)
+{
+    R[0] = W[0];
+    R[1] = c * W[1] + s * W[2];
+    R[2] = -1.0 * s * W[1] + c * W[2];
+    R[3] = W[3];
+}
+
+/* GodunovScheme::operator() (scheme/godunov/godunov.cpp:31-77) with the
+ * boundary states of BoundaryConditions (scheme/boundary/boundary.cpp:29-69)
+ * and ForwardEulerODESolver (scheme/ode/forward_euler/forward_euler.cpp:16-30). */
+void hfo_uns_advance(hfo_uns* m, double dt)
+{
+    const int n = m->n;
+    for (int c = 0; c < n; c++) {
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        const double* U = m->U + 4 * c;
+        for (int e = 0; e < 4; e++) {
+            const int nb = m->nbr[4 * c + e];
+            const double* cs = m->cs + 16 * c + 4 * e;
+            double V[4], rU[4], rV[4], F[4], f[4];
+            if (nb >= 0) {
+                memcpy(V, m->U + 4 * nb, sizeof V);
+            } else if (nb == -1) {                  /* reflective */
+                double t[4];
+                uns_rotate(cs[0], cs[1], U, t);
+                t[1] *= -1.0;
+                uns_rotate(cs[2], cs[3], t, V);
+            } else if (nb == -2) {                  /* transmissive */
+                memcpy(V, U, sizeof V);
+            } else {                                /* inlet */
+                memcpy(V, m->inlet, sizeof V);
+            }
+            uns_rotate(cs[0], cs[1], U, rU);
+            uns_rotate(cs[0], cs[1], V, rV);
+            hfo_hllc(m->gamma, rU, rV, 0, F);
+            uns_rotate(cs[2], cs[3], F, f);
+            for (int k = 0; k < 4; k++) {
+                f[k] *= m->len[4 * c + e];
+                acc[k] += f[k];
+            }
+        }
+        const double iv = m->inv_vol[c];
+        for (int k = 0; k < 4; k++) {
+            acc[k] *= iv;
+            acc[k] *= -1.0;
+            m->L[4 * c + k] = acc[k];
+        }
+    }
+    for (int i = 0; i < 4 * n; i++) {
+        double x = m->L[i];
+        x *= dt;
+        x += m->U[i];
+        m->U[i] = x;
+    }
 }

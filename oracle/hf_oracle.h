@@ -56,6 +56,21 @@ int hfo_run(hfo_mesh* m, int nsteps, double* dt_steps, double* dt_calls, int max
 /* The log of every TimeStep call made so far (step and predictor calls). */
 int hfo_dt_log(const hfo_mesh* m, double* out, int max);
 
+/* ---- unstructured quadrilateral-cell Godunov solver (src/fvm_2d_uns) ---- */
+typedef struct hfo_uns hfo_uns;
+/* vertices [n][4][2] (counter-clockwise), codes [n][4] (0 interior, -1
+ * reflective, -2 transmissive, -3 inlet); inlet: conserved state. */
+hfo_uns* hfo_uns_create(int ncells, const double* vertices, const int* codes, double gamma, double cfl,
+                        const double* inlet);
+void hfo_uns_destroy(hfo_uns* m);
+/* Neighbour cell or boundary code of every cell edge [n][4]. */
+void hfo_uns_connectivity(const hfo_uns* m, int* out);
+void hfo_uns_geometry(const hfo_uns* m, double* centroids, double* volumes);
+void hfo_uns_set_fields(hfo_uns* m, const double* U);   /* [n][4] */
+void hfo_uns_get_fields(const hfo_uns* m, double* U);
+double hfo_uns_time_step(hfo_uns* m);                   /* counted TimeStep call */
+void hfo_uns_advance(hfo_uns* m, double dt);            /* ForwardEuler(GodunovScheme) */
+
 /* Point-wise pieces, for unit tests. */
 void hfo_cons_to_prim(double gamma, const double* U, double* W);
 void hfo_prim_to_cons(double gamma, const double* W, double* U);

@@ -222,3 +222,118 @@ def read_ref_output(path):
     off += 8 * ns
     (elapsed,) = take("<d")
     return fields, calls, steps, elapsed
+
+
+# ------------------------------------------- unstructured solver (fvm_2d_uns) --
+REF_UNS_HARNESS = os.path.join(HERE, "_ref", "ref_uns_harness")
+_uns_bound = False
+
+
+def _uns_lib():
+    global _uns_bound
+    L = lib()
+    if not _uns_bound:
+        L.hfo_uns_create.restype = C.c_void_p
+        L.hfo_uns_create.argtypes = [C.c_int, _dp, _ip, C.c_double, C.c_double, _dp]
+        L.hfo_uns_destroy.argtypes = [C.c_void_p]
+        L.hfo_uns_connectivity.argtypes = [C.c_void_p, _ip]
+        L.hfo_uns_geometry.argtypes = [C.c_void_p, _dp, _dp]
+        L.hfo_uns_set_fields.argtypes = [C.c_void_p, _dp]
+        L.hfo_uns_get_fields.argtypes = [C.c_void_p, _dp]
+        L.hfo_uns_time_step.argtypes = [C.c_void_p]
+        L.hfo_uns_time_step.restype = C.c_double
+        L.hfo_uns_advance.argtypes = [C.c_void_p, C.c_double]
+        _uns_bound = True
+    return L
+
+
+class UnsOracle:
+    """The C restatement of the reference's unstructured Godunov solver."""
+
+    def __init__(self, quads, codes, inlet_cons, gamma=1.4, cfl=0.5):
+        L = _uns_lib()
+        self._q = np.ascontiguousarray(quads, np.float64).reshape(-1)
+        self._c = np.ascontiguousarray(codes, np.int32).reshape(-1)
+        self.n = len(self._q) // 8
+        self._h = L.hfo_uns_create(self.n, self._q, self._c, gamma, cfl,
+                                   np.ascontiguousarray(inlet_cons, np.float64))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _uns_lib().hfo_uns_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def connectivity(self):
+        out = np.empty((self.n, 4), np.int32)
+        _uns_lib().hfo_uns_connectivity(self._h, out)
+        return out
+
+    def geometry(self):
+        cen = np.empty((self.n, 2))
+        vol = np.empty(self.n)
+        _uns_lib().hfo_uns_geometry(self._h, cen, vol)
+        return cen, vol
+
+    def set_fields(self, U):
+        _uns_lib().hfo_uns_set_fields(self._h, np.ascontiguousarray(U, np.float64))
+
+    def fields(self):
+        out = np.empty((self.n, 4))
+        _uns_lib().hfo_uns_get_fields(self._h, out)
+        return out
+
+    def time_step(self):
+        return _uns_lib().hfo_uns_time_step(self._h)
+
+    def advance(self, dt):
+        _uns_lib().hfo_uns_advance(self._h, dt)
+
+    def run(self, nsteps):
+        dts = []
+        for _ in range(nsteps):
+            dt = self.time_step()
+            self.advance(dt)
+            dts.append(dt)
+        return np.array(dts)
+
+
+def have_uns_reference() -> bool:
+    return os.path.exists(REF_UNS_HARNESS)
+
+
+def run_uns_reference(case_json: dict, nsteps: int, timeout: float = 3600):
+    """Run the reference's unstructured solver (oracle/_ref/ref_uns_harness)
+    on a case dict whose mesh.file is an absolute .hmsh path.  Returns a dict
+    with connectivity [n][4], centroids, volumes, fields [n][4], dts, elapsed."""
+    import json
+    if not have_uns_reference():
+        raise RuntimeError("reference unstructured harness not built")
+    with tempfile.TemporaryDirectory() as td:
+        cj = os.path.join(td, "case.json")
+        out = os.path.join(td, "out.bin")
+        with open(cj, "w") as f:
+            json.dump(case_json, f)
+        subprocess.run([REF_UNS_HARNESS, cj, str(nsteps), out], check=True, stdout=subprocess.DEVNULL,
+                       timeout=timeout)
+        data = open(out, "rb").read()
+    nc, ns = struct.unpack_from("<ii", data, 0)
+    off = 8
+    rec = np.frombuffer(data, np.int32, nc * 12, off).reshape(nc, 4, 3)
+    off += nc * 48
+    cen = np.frombuffer(data, np.float64, nc * 2, off).reshape(nc, 2).copy()
+    off += nc * 16
+    vol = np.frombuffer(data, np.float64, nc, off).copy()
+    off += nc * 8
+    U = np.frombuffer(data, np.float64, nc * 4, off).reshape(nc, 4).copy()
+    off += nc * 32
+    dts = np.frombuffer(data, np.float64, ns, off).copy()
+    off += ns * 8
+    (elapsed,) = struct.unpack_from("<d", data, off)
+    return {"connectivity": rec[:, :, 1].copy(), "centroids": cen, "volumes": vol, "fields": U, "dts": dts,
+            "elapsed": elapsed}

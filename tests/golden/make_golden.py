@@ -7,6 +7,11 @@ oracle/Makefile).  Run in a checkout where the reference is present:
 Each fixture is an .npz with the case JSON (as text), the initial padded
 fields, the fields after `nsteps` steps and every TimeStep result.  Only data
 is stored; no reference source.
+
+    python tests/golden/make_golden.py --cli     # CSV frames of the reference program
+    python tests/golden/make_golden.py --uns     # unstructured solver, small meshes
+    python tests/golden/make_golden.py --kumar   # shipped scramjet case, coarse mesh
+    python tests/golden/make_golden.py --uns-cli # CSV frames of the reference's unstructured program
 """
 import json
 import os
@@ -86,7 +91,7 @@ def main():
         print("wrote", name)
 
 
-if __name__ == "__main__":
+if __name__ == "__main__" and len(sys.argv) == 1:
     main()
 
 
@@ -119,3 +124,141 @@ def cli_goldens():
 
 if __name__ == "__main__" and "--cli" in sys.argv:
     cli_goldens()
+
+
+# --------------------------------------------------- unstructured solver --
+KUMAR = "/root/reference/cases/fvm_2d_uns/scramjet_kumar"
+
+
+def uns_cases():
+    """(name, quads, codes, case JSON without the mesh file, nsteps) of the
+    unstructured goldens: a compression-ramp channel with inlet,
+    transmissive and reflective walls; a sheared box with a blast
+    (non-orthogonal cells, reflective walls); a transmissive box."""
+    from hyperflow_amd import uns as U
+    out = []
+    q, c = U.ramp_channel(30, 12)
+    inflow = [1.4, 3.0, 0.0, 1.0]
+    out.append(("uns_ramp_channel_30x12", q, c,
+                U.case_json("", (0.0, 3.0, 0.0, 1.0), inflow,
+                            {"type": "Constant", "variables": "Primitive", "init_state": inflow}, cfl=0.5), 40))
+    q, c = U.box(16, 16, code=U.REFLECTIVE, skew=0.3)
+    out.append(("uns_skewed_box_blast_16x16", q, c,
+                U.case_json("", (0.0, 1.0, 0.0, 1.0), [1.0, 0.0, 0.0, 1.0],
+                            {"type": "SphericalRiemannProblem", "radius": 0.2, "x_origin": 0.4, "y_origin": 0.55,
+                             "variables": "Primitive", "init_sphere_state": [1.0, 0.0, 0.0, 10.0],
+                             "init_ext_state": [0.125, 0.0, 0.0, 0.1]}, cfl=0.8), 40))
+    q, c = U.box(12, 10, code=U.TRANSMISSIVE)
+    out.append(("uns_transmissive_box_12x10", q, c,
+                U.case_json("", (0.0, 1.0, 0.0, 1.0), [1.0, 0.0, 0.0, 1.0],
+                            {"type": "SphericalRiemannProblem", "radius": 0.3, "x_origin": 0.5, "y_origin": 0.5,
+                             "variables": "Primitive", "init_sphere_state": [1.0, 0.5, -0.25, 2.0],
+                             "init_ext_state": [0.5, 0.0, 0.0, 0.4]}, cfl=0.5), 30))
+    return out
+
+
+def uns_goldens():
+    """Goldens of the reference's unstructured solver
+    (oracle/_ref/ref_uns_harness): full arrays for small meshes."""
+    import tempfile
+    from hyperflow_amd import uns as U
+    if not hfo.have_uns_reference():
+        sys.exit("oracle/_ref/ref_uns_harness is not built (needs the reference checkout)")
+    for name, q, c, cj, n in uns_cases():
+        with tempfile.TemporaryDirectory() as td:
+            mf = os.path.join(td, "mesh.hmsh")
+            U.write_hmsh(mf, q, c)
+            cj["mesh"]["file"] = mf
+            r0 = hfo.run_uns_reference(cj, 0)
+            r = hfo.run_uns_reference(cj, n)
+        cj["mesh"]["file"] = "mesh.hmsh"
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), case_json=np.array(json.dumps(cj)), quads=q,
+                            codes=c, connectivity=r["connectivity"], centroids=r["centroids"],
+                            volumes=r["volumes"], init=r0["fields"], fields=r["fields"], dts=r["dts"], steps=np.array(n))
+        print("wrote", name)
+
+
+def kumar_golden(nsteps=20):
+    """The shipped scramjet case (scramjet_kumar_godunov_hllc.json) on the
+    coarse mesh of scramjet_kumar_coarse_mesh.json: SHA-256 of the .hmsh
+    text that the reference's mesh generator writes, of the reference
+    solver's connectivity and of its fields after nsteps, plus the TimeStep
+    results and a sample of cells.  The reference's mesh loader searches
+    neighbours over all cell pairs, so this takes several minutes."""
+    import hashlib
+    import importlib.util
+    import tempfile
+    spec = importlib.util.spec_from_file_location(
+        "generate_block_mesh", "/root/reference/src/mesh_generation/generate_block_mesh.py")
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    cj = json.load(open(os.path.join(KUMAR, "scramjet_kumar_godunov_hllc.json")))
+    with tempfile.TemporaryDirectory() as td:
+        mf = os.path.join(td, "kumar.hmsh")
+        gen.BlockStructuredMeshGenerator(os.path.join(KUMAR, "scramjet_kumar_coarse_mesh.json")).generate_mesh(mf)
+        text = open(mf, "rb").read()
+        cj["mesh"]["file"] = mf
+        r = hfo.run_uns_reference(cj, nsteps)
+    rng = np.random.default_rng(7)
+    idx = np.sort(rng.choice(len(r["fields"]), 64, replace=False))
+    out = {"mesh_json": "scramjet_kumar_coarse_mesh.json", "case_json": "scramjet_kumar_godunov_hllc.json",
+           "hmsh_sha256": hashlib.sha256(text).hexdigest(), "hmsh_bytes": len(text),
+           "ncells": int(len(r["fields"])), "nsteps": nsteps,
+           "connectivity_sha256": hashlib.sha256(np.ascontiguousarray(r["connectivity"], "<i4").tobytes()).hexdigest(),
+           "fields_sha256": hashlib.sha256(np.ascontiguousarray(r["fields"], "<f8").tobytes()).hexdigest(),
+           "dts_hex": [float(v).hex() for v in r["dts"]],
+           "sample_cells": idx.tolist(), "sample_fields_hex": [[float(v).hex() for v in r["fields"][i]] for i in idx],
+           "reference_seconds_per_step": r["elapsed"] / nsteps,
+           "case": json.load(open(os.path.join(KUMAR, "scramjet_kumar_godunov_hllc.json"))),
+           "mesh_blocks": json.load(open(os.path.join(KUMAR, "scramjet_kumar_coarse_mesh.json")))}
+    with open(os.path.join(HERE, "uns_kumar_coarse.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    print("wrote uns_kumar_coarse.json")
+
+
+def uns_cli_goldens():
+    """Per-frame SHA-256 of the results_%06d.csv files the reference's
+    unstructured program (oracle/_ref/ref_uns_cli, its own main.cpp) writes
+    for small cases, the mesh written next to the case file."""
+    import hashlib
+    import subprocess
+    import tempfile
+    from hyperflow_amd import uns as U
+    ref = os.path.join(ROOT, "oracle", "_ref", "ref_uns_cli")
+    out = {}
+    q, c = U.ramp_channel(30, 12)
+    inflow = [1.4, 3.0, 0.0, 1.0]
+    cases = [("ramp_channel_30x12", q, c,
+              U.case_json("mesh.hmsh", (0.0, 3.0, 0.0, 1.0), inflow,
+                          {"type": "Constant", "variables": "Primitive", "init_state": inflow}, cfl=0.5,
+                          t_end=0.25)),
+             ("skewed_box_blast_16x16",) + U.box(16, 16, code=U.REFLECTIVE, skew=0.3) + (
+                 U.case_json("mesh.hmsh", (0.0, 1.0, 0.0, 1.0), [1.0, 0.0, 0.0, 1.0],
+                             {"type": "SphericalRiemannProblem", "radius": 0.2, "x_origin": 0.4, "y_origin": 0.55,
+                              "variables": "Primitive", "init_sphere_state": [1.0, 0.0, 0.0, 10.0],
+                              "init_ext_state": [0.125, 0.0, 0.0, 0.1]}, cfl=0.8, t_end=0.1),)]
+    for name, q, c, cj in cases:
+        cj["simulation"]["animation"] = {"duration": 1.0, "fps": 10.0}
+        with tempfile.TemporaryDirectory() as td:
+            U.write_hmsh(os.path.join(td, "mesh.hmsh"), q, c)
+            with open(os.path.join(td, "case.json"), "w") as f:
+                json.dump(cj, f)
+            r = subprocess.run([ref, "case.json"], cwd=td, stdout=subprocess.PIPE, text=True, check=True)
+            steps = sum(1 for ln in r.stdout.splitlines() if ln.startswith("Steps:"))
+            frames = sorted(f for f in os.listdir(td) if f.startswith("results_"))
+            hashes = {f: hashlib.sha256(open(os.path.join(td, f), "rb").read()).hexdigest() for f in frames}
+            last = open(os.path.join(td, frames[-1])).read()
+        out[name] = {"case": cj, "quads": q.tolist(), "codes": c.tolist(), "progress_lines": steps,
+                     "frames": hashes, "last_frame_head": "\n".join(last.splitlines()[:12])}
+        print(name, steps, len(frames))
+    with open(os.path.join(HERE, "uns_cli_frames.json"), "w") as f:
+        json.dump(out, f, sort_keys=True)
+    print("wrote uns_cli_frames.json")
+
+
+if __name__ == "__main__" and "--uns-cli" in sys.argv:
+    uns_cli_goldens()
+if __name__ == "__main__" and "--uns" in sys.argv:
+    uns_goldens()
+if __name__ == "__main__" and "--kumar" in sys.argv:
+    kumar_golden()

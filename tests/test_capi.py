@@ -4,10 +4,12 @@ import ctypes as C
 import os
 import re
 
+import numpy as np
 import pytest
 
 from hyperflow_amd import native
 from hyperflow_amd import case as K
+from hyperflow_amd import uns
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -28,8 +30,41 @@ def test_library_exports_every_declared_entry_point():
     for n in learned:
         assert hasattr(lib, n), n
     assert set(learned) == set(native.EXPORTS_LEARNED)
+    unstructured = header_functions("hyperflow_uns.h")
+    for n in unstructured:
+        assert hasattr(lib, n), n
+    assert set(unstructured) == set(uns.EXPORTS_UNS)
     headers = sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h"))
-    assert headers == ["hyperflow.h", "hyperflow_learned.h"]
+    assert headers == ["hyperflow.h", "hyperflow_learned.h", "hyperflow_uns.h"]
+
+
+def test_uns_desc_layout():
+    assert C.sizeof(uns.hf_uns_desc) == 2 * 4 + 2 * 8 + 2 * 8 + 4 * 8
+
+
+@pytest.mark.parametrize("over,msg", [
+    ({"ncells": 0}, "no cells"),
+    ({"cfl": 0.0}, "cfl"),
+    ({"gamma": float("nan")}, "gamma"),
+    ({"code": 5}, "boundary code 5"),
+])
+def test_invalid_uns_descriptors_are_rejected(over, msg):
+    q, c = uns.box(3, 2)
+    c = np.ascontiguousarray(c, np.int32)
+    if "code" in over:
+        c[1, 2] = over.pop("code")
+    v = np.ascontiguousarray(q, np.float64)
+    d = uns.hf_uns_desc()
+    d.ncells, d.gamma, d.cfl = len(q), 1.4, 0.5
+    d.vertices = v.ctypes.data_as(C.POINTER(C.c_double))
+    d.edge_types = c.ctypes.data_as(C.POINTER(C.c_int32))
+    for k, val in over.items():
+        setattr(d, k, val)
+    h = C.c_void_p()
+    rc = uns._lib().hf_uns_create(C.byref(d), C.byref(h))
+    assert rc == -1
+    assert msg in native.lib().hf_last_error().decode()
+    assert not h.value
 
 
 def test_net_desc_layout_and_param_count():
