@@ -28,7 +28,7 @@ struct hf_uns {
     double* d_dr = nullptr;
     double* buf[2] = {nullptr, nullptr};
     int cur = 0;
-    unsigned long long* d_slots = nullptr;   // [3]
+    unsigned long long* d_slots = nullptr;   // [3][hfu::SLOT_WORDS]
     int slot = -1;                           // slot holding buf[cur]'s minimum, or -1
     double* d_hist = nullptr;
     int hist_cap = 1 << 20;
@@ -237,7 +237,8 @@ int hf_uns_create(const hf_uns_desc* d, hf_uns** out)
     HF_HIPU(hipMalloc(&U->d_dr, sizeof(double) * n));
     HF_HIPU(hipMalloc(&U->buf[0], sizeof(double) * 4 * n));
     HF_HIPU(hipMalloc(&U->buf[1], sizeof(double) * 4 * n));
-    HF_HIPU(hipMalloc(&U->d_slots, sizeof(unsigned long long) * 3));
+    HF_HIPU(hipMalloc(&U->d_slots, sizeof(unsigned long long) * 3 * hfu::SLOT_WORDS));
+    HF_HIPU(hipMemset(U->d_slots, 0, sizeof(unsigned long long) * 3 * hfu::SLOT_WORDS));
     HF_HIPU(hipMalloc(&U->d_hist, sizeof(double) * U->hist_cap));
     HF_HIPU(hipMemcpy(U->d_nbr, nbr_em.data(), sizeof(int32_t) * 4 * n, hipMemcpyHostToDevice));
     HF_HIPU(hipMemcpy(U->d_len, len.data(), sizeof(double) * 4 * n, hipMemcpyHostToDevice));
@@ -320,9 +321,12 @@ int hf_uns_time_step(hf_uns* u, double* dt)
     if (!u || !dt) return fail(HF_EINVAL, "hf_uns_time_step: null argument");
     int rc = ensure_slot(u);
     if (rc) return rc;
-    unsigned long long b = 0;
-    HF_HIP(hipMemcpyAsync(&b, u->d_slots + u->slot, sizeof b, hipMemcpyDeviceToHost, u->stream));
+    std::vector<unsigned long long> w(hfu::SLOT_WORDS);
+    HF_HIP(hipMemcpyAsync(w.data(), u->d_slots + (size_t)u->slot * hfu::SLOT_WORDS, sizeof(unsigned long long) * w.size(),
+                          hipMemcpyDeviceToHost, u->stream));
     HF_HIP(hipStreamSynchronize(u->stream));
+    unsigned long long b = w[0];
+    for (int k = 1; k < hfu::SLOT_WAYS; k++) b = std::min(b, w[(size_t)k * hfu::SLOT_STRIDE]);
     double m;
     std::memcpy(&m, &b, sizeof m);
     const int64_t k = ++u->calls;
@@ -357,9 +361,9 @@ int hf_uns_run(hf_uns* u, int nsteps, int flags)
     if (rc) return rc;
     // slots rotate: step reads rd, writes (rd+1)%3 (reset by the previous
     // step or here), resets (rd+2)%3 for the step after
-    const unsigned long long big = 0x4341C37937E08000ull;   // bits of 1e16
-    HF_HIP(hipMemcpyAsync(u->d_slots + (u->slot + 1) % 3, &big, sizeof big, hipMemcpyHostToDevice, u->stream));
-    HF_HIP(hipStreamSynchronize(u->stream));   // the host value above is a stack variable
+    static const std::vector<unsigned long long> empty(hfu::SLOT_WORDS, hfu::SLOT_EMPTY);
+    HF_HIP(hipMemcpyAsync(u->d_slots + (size_t)((u->slot + 1) % 3) * hfu::SLOT_WORDS, empty.data(),
+                          sizeof(unsigned long long) * empty.size(), hipMemcpyHostToDevice, u->stream));
     for (int i = 0; i < nsteps; i++) {
         hfu::UnsStep s = step_args(u);
         s.rd = u->slot;

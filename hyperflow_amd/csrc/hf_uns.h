@@ -19,14 +19,24 @@ struct UnsMesh {
     double inlet[4];
 };
 
-// One step: U^{n+1} = B from U^n = A.  The dt comes from slots[rd] (the
-// minimum of dr / (|u| + c) over the cells, as bits of a positive double)
-// unless use_fixed; the new state's minimum goes to slots[wr] (wr < 0: not
-// computed) and slots[rs] is reset for the step after next.
+// A slot holds the minimum of dr / (|u| + c) over the cells as the bits of a
+// positive double, spread over SLOT_WAYS words on separate 128-byte lines:
+// each workgroup reduces its cells and does one atomicMin on word
+// (workgroup % SLOT_WAYS); a reader takes the minimum of the words.  Same-
+// address atomics from every workgroup serialise at the memory side; spread
+// over lines they proceed in parallel.
+constexpr int SLOT_WAYS = 64;
+constexpr int SLOT_STRIDE = 16;                       // u64 words between ways (128 B)
+constexpr int SLOT_WORDS = SLOT_WAYS * SLOT_STRIDE;   // one slot
+constexpr unsigned long long SLOT_EMPTY = 0x4341C37937E08000ull;   // bits of 1e16 (timestep.cpp:37)
+
+// One step: U^{n+1} = B from U^n = A.  The dt comes from slot rd unless
+// use_fixed; the new state's minimum goes to slot wr (wr < 0: not computed)
+// and slot rs is reset for the step after next.
 struct UnsStep {
     const double* A;
     double* B;
-    unsigned long long* slots;
+    unsigned long long* slots;   // [3][SLOT_WORDS]
     int32_t rd, wr, rs, use_fixed;
     double fixed_dt;
     long long kcall;         // TimeStep call index (1-based) of this step
@@ -35,7 +45,7 @@ struct UnsStep {
 };
 
 hipError_t launch_uns_step(const UnsMesh& m, const UnsStep& s, hipStream_t st);
-// minimum of dr / (|u| + c) of state A into slots[slot] (reset first)
+// minimum of dr / (|u| + c) of state A into slot `slot` (reset first)
 hipError_t launch_uns_scan(const UnsMesh& m, const double* A, unsigned long long* slots, int slot, hipStream_t st);
 
 }  // namespace hfu

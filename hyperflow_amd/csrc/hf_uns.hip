@@ -45,6 +45,21 @@ __device__ __forceinline__ unsigned long long wave_min_bits(unsigned long long b
     return b;
 }
 
+// The workgroup's minimum into one way of a slot: wave minima through LDS,
+// then one atomic.  Called by every thread of the workgroup.
+__device__ __forceinline__ void block_min_to_slot(unsigned long long cand, unsigned long long* slot)
+{
+    __shared__ unsigned long long wmin[4];
+    cand = wave_min_bits(cand);
+    if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = cand;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long b = wmin[0];
+        for (int w = 1; w < 4; w++) b = wmin[w] < b ? wmin[w] : b;
+        atomicMin(slot + (blockIdx.x % SLOT_WAYS) * SLOT_STRIDE, b);
+    }
+}
+
 // GodunovScheme::operator() (scheme/godunov/godunov.cpp:31-77), boundary
 // states of BoundaryConditions (scheme/boundary/boundary.cpp:29-69), and
 // ForwardEulerODESolver (scheme/ode/forward_euler/forward_euler.cpp:16-30).
@@ -53,16 +68,17 @@ __global__ __launch_bounds__(256) void k_uns_step(const UnsMesh m, const UnsStep
     const int c = blockIdx.x * 256 + threadIdx.x;
     const int n = m.n;
     const double gamma = m.gamma, g1 = m.gamma - 1.0;
+    const int lane = threadIdx.x & 63;
     double dt;
     if (s.use_fixed) {
         dt = s.fixed_dt;
     } else {
         const double cf = (s.kcall <= 10) ? 0.2 * m.cfl : m.cfl;   // timestep.cpp:57-61
-        dt = cf * double_of(s.slots[s.rd]);
+        dt = cf * double_of(wave_min_bits(s.slots[s.rd * SLOT_WORDS + lane * SLOT_STRIDE]));
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        if (!s.use_fixed && s.kcall >= 1 && s.kcall <= s.hist_cap) s.hist[s.kcall - 1] = dt;
-        if (s.rs >= 0) s.slots[s.rs] = bits_of(1e16);
+    if (blockIdx.x == 0 && threadIdx.x < 64) {
+        if (threadIdx.x == 0 && !s.use_fixed && s.kcall >= 1 && s.kcall <= s.hist_cap) s.hist[s.kcall - 1] = dt;
+        if (s.rs >= 0) s.slots[s.rs * SLOT_WORDS + lane * SLOT_STRIDE] = SLOT_EMPTY;
     }
     unsigned long long cand = ~0ull;
     if (c < n) {
@@ -119,10 +135,7 @@ __global__ __launch_bounds__(256) void k_uns_step(const UnsMesh m, const UnsStep
         }
         if (s.wr >= 0) cand = bits_of(cell_dt(N, m.dr[c], gamma, g1));
     }
-    if (s.wr >= 0) {
-        cand = wave_min_bits(cand);
-        atomicMin(&s.slots[s.wr], cand);   // every lane holds the wave minimum
-    }
+    if (s.wr >= 0) block_min_to_slot(cand, s.slots + s.wr * SLOT_WORDS);
 }
 
 __global__ __launch_bounds__(256) void k_uns_scan(const UnsMesh m, const double* __restrict__ A,
@@ -137,11 +150,10 @@ __global__ __launch_bounds__(256) void k_uns_scan(const UnsMesh m, const double*
         for (int k = 0; k < 4; k++) U[k] = A[(size_t)k * n + c];
         cand = bits_of(cell_dt(U, m.dr[c], m.gamma, m.gamma - 1.0));
     }
-    cand = wave_min_bits(cand);
-    atomicMin(slot, cand);
+    block_min_to_slot(cand, slot);
 }
 
-__global__ void k_uns_reset(unsigned long long* slot) { *slot = bits_of(1e16); }
+__global__ void k_uns_reset(unsigned long long* slot) { slot[threadIdx.x * SLOT_STRIDE] = SLOT_EMPTY; }
 
 hipError_t launch_uns_step(const UnsMesh& m, const UnsStep& s, hipStream_t st)
 {
@@ -151,8 +163,8 @@ hipError_t launch_uns_step(const UnsMesh& m, const UnsStep& s, hipStream_t st)
 
 hipError_t launch_uns_scan(const UnsMesh& m, const double* A, unsigned long long* slots, int slot, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_uns_reset, dim3(1), dim3(1), 0, st, slots + slot);
-    hipLaunchKernelGGL(k_uns_scan, dim3((m.n + 255) / 256), dim3(256), 0, st, m, A, slots + slot);
+    hipLaunchKernelGGL(k_uns_reset, dim3(1), dim3(SLOT_WAYS), 0, st, slots + slot * SLOT_WORDS);
+    hipLaunchKernelGGL(k_uns_scan, dim3((m.n + 255) / 256), dim3(256), 0, st, m, A, slots + slot * SLOT_WORDS);
     return hipGetLastError();
 }
 
