@@ -273,6 +273,7 @@ template <class R>
 int launch_update_t(hf_ctx* c, HfStepArgs<R>& a, bool resid, bool timed)
 {
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    timed = timed && a.nitems > 0;   // an empty launch is not issued
     if (timed) {
         if (c->ev_free.empty()) {
             HF_HIP(hipEventCreate(&e0));
@@ -282,14 +283,10 @@ int launch_update_t(hf_ctx* c, HfStepArgs<R>& a, bool resid, bool timed)
             e1 = c->ev_free.back().second;
             c->ev_free.pop_back();
         }
-        HF_HIP(hipEventRecord(e0, c->stream));
     }
     const int grid = std::max(1, std::min(c->wg_cap, (a.nitems + 3) / 4));
-    HF_HIP(hf::launch_update<R>(c->scheme, c->limiter, resid, a, grid, c->stream));
-    if (timed) {
-        HF_HIP(hipEventRecord(e1, c->stream));
-        c->ev_pending.emplace_back(e0, e1);
-    }
+    HF_HIP(hf::launch_update<R>(c->scheme, c->limiter, resid, a, grid, c->stream, e0, e1));
+    if (timed) c->ev_pending.emplace_back(e0, e1);
     return HF_OK;
 }
 
@@ -414,7 +411,9 @@ int ensure_partials(hf_ctx* c)
 int enqueue_steps(hf_ctx* c, int nsteps, bool timed, bool fixed = false, double dt = 0.0)
 {
     const int c0 = c->cur;
+    const bool timing = timed;
     for (int s = 0; s < nsteps; s++) {
+        timed = timing && (c->prof_step++ % c->profile == 0);
         const int k = c->cur;
         void* A = c->buf[k];
         void* B = c->buf[k ^ 1];
@@ -802,7 +801,7 @@ int hf_advance(hf_ctx* c, const double* dt)
                 }
             }
             int rc = launch_update_host(c, c->item_begin[q], c->item_count[q], A, B, B, false, dt[p], dt_pred,
-                                        c->profile);
+                                        c->profile != 0);
             if (rc) return rc;
         }
     }
@@ -842,7 +841,7 @@ static int run_loop(hf_ctx* c, int nsteps, int flags, bool fixed, double dt)
         }
         HF_HIP(hipGraphLaunch(it->second.second, c->stream));
     } else {
-        rc = enqueue_steps(c, nsteps, c->profile, fixed, dt);
+        rc = enqueue_steps(c, nsteps, c->profile != 0, fixed, dt);
         if (rc) return rc;
     }
     c->cur ^= (nsteps & 1);
@@ -902,7 +901,9 @@ int hf_calls(hf_ctx* c, int p, int64_t* calls)
 int hf_profile(hf_ctx* c, int enable)
 {
     if (!c) return fail(HF_EINVAL, "null context");
-    c->profile = enable != 0;
+    if (enable < 0) return fail(HF_EINVAL, "hf_profile: period must be >= 0");
+    c->profile = enable;
+    c->prof_step = 0;
     return HF_OK;
 }
 

@@ -71,7 +71,8 @@ struct hf_ctx {
     // captured step loops by (steps, start parity, fixed-dt bits or 0 for the CFL dt)
     std::map<std::tuple<int, int, uint64_t>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
 
-    bool profile = false;
+    int profile = 0;                 // 0 off, else time the launches of every profile-th step
+    int64_t prof_step = 0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending, ev_free;
     int64_t prof_launches = 0;
     double prof_ms = 0.0;

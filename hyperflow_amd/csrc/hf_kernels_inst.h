@@ -4,26 +4,35 @@
 #include "hf_kernels.h"
 #include "hf_launch.h"
 
+#include <hip/hip_ext.h>
+
 namespace hf {
 
 typedef HF_MATH Math;
 typedef Math::R Real;
 
 template <>
-hipError_t launch_update<Real>(int scheme, int limiter, bool resid, const HfStepArgs<Real>& a, int nwg, hipStream_t s)
+hipError_t launch_update<Real>(int scheme, int limiter, bool resid, const HfStepArgs<Real>& a, int nwg, hipStream_t s,
+                              hipEvent_t e0, hipEvent_t e1)
 {
     if (a.nitems <= 0) return hipSuccess;
     const dim3 block(256), grid(nwg);
+#define HF_GO(K)                                                           \
+    do {                                                                   \
+        if (e0 || e1) hipExtLaunchKernelGGL((K), grid, block, 0, s, e0, e1, 0, a); \
+        else hipLaunchKernelGGL((K), grid, block, 0, s, a);                \
+    } while (0)
     if (scheme == 0) {
-        if (resid) hipLaunchKernelGGL((k_godunov<Math, true>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((k_godunov<Math, false>), grid, block, 0, s, a);
+        if (resid) HF_GO((k_godunov<Math, true>));
+        else HF_GO((k_godunov<Math, false>));
     } else if (limiter == 0) {
-        if (resid) hipLaunchKernelGGL((k_muscl<Math, 0, true>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((k_muscl<Math, 0, false>), grid, block, 0, s, a);
+        if (resid) HF_GO((k_muscl<Math, 0, true>));
+        else HF_GO((k_muscl<Math, 0, false>));
     } else {
-        if (resid) hipLaunchKernelGGL((k_muscl<Math, 1, true>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((k_muscl<Math, 1, false>), grid, block, 0, s, a);
+        if (resid) HF_GO((k_muscl<Math, 1, true>));
+        else HF_GO((k_muscl<Math, 1, false>));
     }
+#undef HF_GO
     return hipGetLastError();
 }
 

@@ -8,8 +8,12 @@
 namespace hf {
 
 
+// e0/e1 (optional): events stamped at the dispatch's start and end by the
+// launch itself (hipExtLaunchKernel), for per-launch timing without extra
+// commands on the stream.
 template <class R>
-hipError_t launch_update(int scheme, int limiter, bool resid, const HfStepArgs<R>& a, int grid, hipStream_t s);
+hipError_t launch_update(int scheme, int limiter, bool resid, const HfStepArgs<R>& a, int grid, hipStream_t s,
+                         hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // Workgroups (256 threads) of an update kernel that fit on one CU at once.
 template <class R>
 int update_wgs_per_cu(int scheme, int limiter, bool resid);

@@ -125,7 +125,7 @@ int ensure_slot(hf_uns* u)
     return HF_OK;
 }
 
-// one step launch, bracketed by a pair of events when profiling
+// one step launch, its dispatch stamped by a pair of events when profiling
 hipError_t launch_step(hf_uns* u, const hfu::UnsStep& s)
 {
     if (!u->profile) return hfu::launch_uns_step(u->mesh, s, u->stream);
@@ -138,9 +138,7 @@ hipError_t launch_step(hf_uns* u, const hfu::UnsStep& s)
         e = u->ev_free.back();
         u->ev_free.pop_back();
     }
-    hipError_t r = hipEventRecord(e.first, u->stream);
-    if (r == hipSuccess) r = hfu::launch_uns_step(u->mesh, s, u->stream);
-    if (r == hipSuccess) r = hipEventRecord(e.second, u->stream);
+    const hipError_t r = hfu::launch_uns_step(u->mesh, s, u->stream, e.first, e.second);
     u->ev_pending.push_back(e);
     return r;
 }

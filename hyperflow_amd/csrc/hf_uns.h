@@ -44,7 +44,9 @@ struct UnsStep {
     int32_t hist_cap, pad;
 };
 
-hipError_t launch_uns_step(const UnsMesh& m, const UnsStep& s, hipStream_t st);
+// e0/e1 (optional): events stamped at the dispatch's start and end
+hipError_t launch_uns_step(const UnsMesh& m, const UnsStep& s, hipStream_t st, hipEvent_t e0 = nullptr,
+                           hipEvent_t e1 = nullptr);
 // minimum of dr / (|u| + c) of state A into slot `slot` (reset first)
 hipError_t launch_uns_scan(const UnsMesh& m, const double* A, unsigned long long* slots, int slot, hipStream_t st);
 

@@ -1,6 +1,7 @@
 // hf_uns.hip -- gfx950 kernels of the unstructured quadrilateral-cell
 // Godunov solver (src/fvm_2d_uns), FP64, bit-exact with the reference:
 // compiled with -ffp-contract=off, every expression in the reference's order.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "hf_common.h"
@@ -155,9 +156,10 @@ __global__ __launch_bounds__(256) void k_uns_scan(const UnsMesh m, const double*
 
 __global__ void k_uns_reset(unsigned long long* slot) { slot[threadIdx.x * SLOT_STRIDE] = SLOT_EMPTY; }
 
-hipError_t launch_uns_step(const UnsMesh& m, const UnsStep& s, hipStream_t st)
+hipError_t launch_uns_step(const UnsMesh& m, const UnsStep& s, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
-    hipLaunchKernelGGL(k_uns_step, dim3((m.n + 255) / 256), dim3(256), 0, st, m, s);
+    if (e0 || e1) hipExtLaunchKernelGGL(k_uns_step, dim3((m.n + 255) / 256), dim3(256), 0, st, e0, e1, 0, m, s);
+    else hipLaunchKernelGGL(k_uns_step, dim3((m.n + 255) / 256), dim3(256), 0, st, m, s);
     return hipGetLastError();
 }
 

@@ -250,8 +250,9 @@ class Solver:
         _check(lib().hf_calls(self._h, problem, C.byref(v)), "hf_calls")
         return v.value
 
-    def profile(self, enable: bool) -> None:
-        _check(lib().hf_profile(self._h, int(enable)), "hf_profile")
+    def profile(self, enable, every: int = 1) -> None:
+        """Time the update launches of every `every`-th step (off if not enable)."""
+        _check(lib().hf_profile(self._h, int(every) if enable else 0), "hf_profile")
 
     def kernel_stats(self):
         n = C.c_int64()

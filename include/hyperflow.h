@@ -130,9 +130,11 @@ HF_API int hf_dt_history(hf_ctx* ctx, double* out, int max, int* count);
 /* Number of TimeStep calls made so far by a problem (TimeStep::steps - 1). */
 HF_API int hf_calls(hf_ctx* ctx, int problem, int64_t* calls);
 
-/* Kernel timing with HIP events on the context stream: when enabled, every
- * update-kernel launch of hf_run/hf_advance is bracketed by events; the
- * stats give the number of launches and their summed duration. */
+/* Kernel timing with HIP events stamped by the launches themselves: with
+ * enable = k > 0 the update-kernel launches of every k-th step of hf_run (and
+ * every hf_advance launch) are timed (1 = all; a larger k keeps the timing
+ * cost out of most steps); 0 turns timing off.  The stats give the number of
+ * timed launches and their summed duration. */
 HF_API int hf_profile(hf_ctx* ctx, int enable);
 HF_API int hf_kernel_stats(hf_ctx* ctx, int64_t* launches, double* total_ms);
 /* Compensated (Neumaier) FP64 sums of rho, rho*u, rho*v, E over the interior

@@ -158,3 +158,24 @@ def test_conservation_sums_godunov_reflective_box():
     np.testing.assert_allclose(s1[[0, 3]], s0[[0, 3]], rtol=1e-13)
     exact = np.array([np.sum(np.sort(U[k, 1:-1, 1:-1].ravel())) for U in [s.download(0)] for k in range(4)])
     np.testing.assert_allclose(s1, exact, rtol=1e-14, atol=1e-12)
+
+
+def test_sampled_launch_timing_counts():
+    """hf_profile(k): the update launches of steps 0, k, 2k, ... are timed."""
+    g = goldens.load("step_channel_minbee_8")
+    s = native.Solver(g["case"])
+    s.upload_all(g["init"])
+    say("profile sampling")
+    for every, steps in ((1, 7), (3, 10), (4, 8)):
+        s.profile(True, every=every)
+        s.kernel_stats()
+        s.run(steps)
+        s.synchronize()
+        launches, ms = s.kernel_stats()
+        per_step = len(g["case"].blocks)     # main launch + one fix-up launch per later block
+        assert launches == -(-steps // every) * per_step, (every, steps, launches)
+        assert ms > 0.0
+    s.profile(False)
+    s.run(3)
+    assert s.kernel_stats() == (0, 0.0)
+    s.close()
