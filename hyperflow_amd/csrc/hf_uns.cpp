@@ -76,12 +76,16 @@ bool opposite(const double* a0, const double* a1, const double* b0, const double
 }
 
 // The reference scans every cell edge for each edge (O(n^2)); the same
-// relation through a hash of edge start points on a grid much coarser than
-// the tolerance, searching the neighbouring grid cells, keeping the last
-// match in (cell, edge) order.
+// relation through a hash of edge start points on a grid coarser than the
+// tolerance, searching the neighbouring grid cells, keeping the last match
+// in (cell, edge) order.  Non-finite coordinates are rejected by the caller.
 void connect(int n, const double* V, const int32_t* codes, std::vector<int32_t>& nbr)
 {
-    const double h = 1e-9;
+    // grid spacing: at least the tolerance (a match lies in the same or an
+    // adjacent grid cell), and coarse enough that x / h fits an integer
+    double big = 0.0;
+    for (size_t i = 0; i < (size_t)8 * n; i++) big = std::max(big, std::fabs(V[i]));
+    const double h = std::max(1e-9, big * 0x1p-50);
     auto key = [&](double x, double y) {
         const long long i = (long long)std::floor(x / h), j = (long long)std::floor(y / h);
         return (unsigned long long)(i * 1000003LL) ^ (unsigned long long)(j * 998244353LL);
@@ -164,9 +168,12 @@ int hf_uns_create(const hf_uns_desc* d, hf_uns** out)
     if (!d || !out) return fail(HF_EINVAL, "hf_uns_create: null argument");
     *out = nullptr;
     if (d->ncells < 1 || !d->vertices || !d->edge_types) return fail(HF_EINVAL, "hf_uns_create: no cells");
+    if (d->ncells > (1 << 28)) return fail(HF_EINVAL, "hf_uns_create: more than 2^28 cells");
     if (!std::isfinite(d->gamma) || !std::isfinite(d->cfl) || d->cfl <= 0.0)
         return fail(HF_EINVAL, "hf_uns_create: gamma and cfl must be finite, cfl > 0");
     const int n = d->ncells;
+    for (size_t i = 0; i < (size_t)8 * n; i++)
+        if (!std::isfinite(d->vertices[i])) return fail(HF_EINVAL, "hf_uns_create: non-finite vertex coordinate");
     for (int i = 0; i < 4 * n; i++) {
         const int t = d->edge_types[i];
         if (t != HF_UNS_INTERIOR && t != HF_UNS_REFLECTIVE && t != HF_UNS_TRANSMISSIVE && t != HF_UNS_INLET)

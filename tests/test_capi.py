@@ -47,6 +47,7 @@ def test_uns_desc_layout():
     ({"cfl": 0.0}, "cfl"),
     ({"gamma": float("nan")}, "gamma"),
     ({"code": 5}, "boundary code 5"),
+    ({"nan_vertex": True}, "non-finite vertex"),
 ])
 def test_invalid_uns_descriptors_are_rejected(over, msg):
     q, c = uns.box(3, 2)
@@ -54,6 +55,8 @@ def test_invalid_uns_descriptors_are_rejected(over, msg):
     if "code" in over:
         c[1, 2] = over.pop("code")
     v = np.ascontiguousarray(q, np.float64)
+    if over.pop("nan_vertex", False):
+        v[2, 1, 0] = np.nan
     d = uns.hf_uns_desc()
     d.ncells, d.gamma, d.cfl = len(q), 1.4, 0.5
     d.vertices = v.ctypes.data_as(C.POINTER(C.c_double))
