@@ -131,9 +131,6 @@ def traffic_from_profiles(workload: str):
         return None
 
 
-UNS_BYTES_PER_CELL = 64 + 192   # state in + out; neighbours, lengths, 16 rotation terms, 1/V, shortest edge
-
-
 def uns_line(steps: int, warmup: int, refine: int, with_cpu: bool):
     """The unstructured solver on the shipped scramjet case: its coarse block
     mesh (the reference's scramjet_kumar_coarse_mesh.json, kept with the
@@ -151,6 +148,7 @@ def uns_line(steps: int, warmup: int, refine: int, with_cpu: bool):
     gamma, cfl = case["model"]["gamma"], case["simulation"]["cfl"]
     s = U.UnsSolver(q, c, U.prim_to_cons(case["mesh"]["boundaries"]["inlet"], gamma), gamma, cfl)
     s.upload(U.initial_fields(q, case["initial_condition"], gamma))
+    bpc = s.bytes_per_cell()   # state in + out, neighbour indices, lengths, rotation terms, 1/V
     s.run(warmup)
     s.synchronize()
     s.profile(True)
@@ -163,7 +161,7 @@ def uns_line(steps: int, warmup: int, refine: int, with_cpu: bool):
     s.close()
     n = len(q)
     avg_ms = kernel_ms / launches
-    achieved = n * UNS_BYTES_PER_CELL / (avg_ms * 1e-3) / 1e9
+    achieved = n * bpc / (avg_ms * 1e-3) / 1e9
     line = {
         "metric": "unstructured Godunov-HLLC Mcell-updates/s", "value": round(n * steps / wall / 1e6, 2),
         "unit": "Mcell-updates/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
@@ -175,7 +173,7 @@ def uns_line(steps: int, warmup: int, refine: int, with_cpu: bool):
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(f"u1_r{refine}"),
                      "kernel": "k_uns_step", "avg_launch_ms": round(avg_ms, 5), "launches": launches,
-                     "algorithmic_bytes_per_launch": n * UNS_BYTES_PER_CELL, "bytes_per_cell": UNS_BYTES_PER_CELL},
+                     "algorithmic_bytes_per_launch": n * bpc, "bytes_per_cell": bpc},
     }
     if with_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

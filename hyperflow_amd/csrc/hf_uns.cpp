@@ -25,7 +25,6 @@ struct hf_uns {
     double* d_len = nullptr;
     double* d_cs = nullptr;
     double* d_iv = nullptr;
-    double* d_dr = nullptr;
     double* buf[2] = {nullptr, nullptr};
     int cur = 0;
     unsigned long long* d_slots = nullptr;   // [3][hfu::SLOT_WORDS]
@@ -44,7 +43,12 @@ namespace {
 
 int fail(int code, const std::string& msg) { return hf_fail(code, msg); }
 
-inline double smin(double a, double b) { return (b < a) ? b : a; }
+inline uint64_t bits(double x)
+{
+    uint64_t b;
+    std::memcpy(&b, &x, sizeof b);
+    return b;
+}
 
 // Edge(start, end) (geometry/edge/edge.cpp:12-238): length, and the cosine
 // and sine of the outward normal angle and of its negative
@@ -187,7 +191,8 @@ int hf_uns_create(const hf_uns_desc* d, hf_uns** out)
     for (int k = 0; k < 4; k++) u->inlet[k] = d->inlet[k];
 
     const double* V = d->vertices;
-    std::vector<double> len((size_t)4 * n), cs((size_t)16 * n), iv(n), dr(n);
+    std::vector<double> len((size_t)4 * n), cs((size_t)16 * n), iv(n);
+    bool mirrored = true;   // every edge's back rotation is (cos, -sin) of the forward one, bit for bit
     u->cen.resize((size_t)2 * n);
     u->vol.resize(n);
     for (int c = 0; c < n; c++) {
@@ -210,15 +215,15 @@ int hf_uns_create(const hf_uns_desc* d, hf_uns** out)
         u->cen[2 * c] = cx / (6.0 * vol);
         u->cen[2 * c + 1] = cy / (6.0 * vol);
         iv[c] = 1.0 / vol;   // godunov.cpp:71
-        double shortest = 1e16;
+        // the incircle radius (shortest edge, polygon.cpp:79-91) is taken from
+        // these lengths on the device
         for (int e = 0; e < 4; e++) {
             double l, r[4];
             edge_geometry(q + 2 * e, q + 2 * ((e + 1) % 4), l, r);
             len[(size_t)e * n + c] = l;
             for (int k = 0; k < 4; k++) cs[(size_t)(4 * e + k) * n + c] = r[k];
-            shortest = (shortest < l) ? shortest : l;   // std::min(edge length, shortest), polygon.cpp:79-91
+            mirrored = mirrored && bits(r[2]) == bits(r[0]) && bits(r[3]) == bits(-r[1]);
         }
-        dr[c] = shortest;
     }
     connect(n, V, d->edge_types, u->nbr);
     std::vector<int32_t> nbr_em((size_t)4 * n);
@@ -237,9 +242,15 @@ int hf_uns_create(const hf_uns_desc* d, hf_uns** out)
     HF_HIPU(hipStreamCreateWithFlags(&U->stream, hipStreamNonBlocking));
     HF_HIPU(hipMalloc(&U->d_nbr, sizeof(int32_t) * 4 * n));
     HF_HIPU(hipMalloc(&U->d_len, sizeof(double) * 4 * n));
-    HF_HIPU(hipMalloc(&U->d_cs, sizeof(double) * 16 * n));
+    // the device keeps 2 rotation terms per edge when the back rotation
+    // mirrors the forward one (glibc's sin is odd and cos even), else all 4
+    const int terms = mirrored ? 2 : 4;
+    std::vector<double> cst((size_t)4 * terms * n);
+    for (int e = 0; e < 4; e++)
+        for (int k = 0; k < terms; k++)
+            std::memcpy(&cst[(size_t)(terms * e + k) * n], &cs[(size_t)(4 * e + k) * n], sizeof(double) * n);
+    HF_HIPU(hipMalloc(&U->d_cs, sizeof(double) * cst.size()));
     HF_HIPU(hipMalloc(&U->d_iv, sizeof(double) * n));
-    HF_HIPU(hipMalloc(&U->d_dr, sizeof(double) * n));
     HF_HIPU(hipMalloc(&U->buf[0], sizeof(double) * 4 * n));
     HF_HIPU(hipMalloc(&U->buf[1], sizeof(double) * 4 * n));
     HF_HIPU(hipMalloc(&U->d_slots, sizeof(unsigned long long) * 3 * hfu::SLOT_WORDS));
@@ -247,9 +258,8 @@ int hf_uns_create(const hf_uns_desc* d, hf_uns** out)
     HF_HIPU(hipMalloc(&U->d_hist, sizeof(double) * U->hist_cap));
     HF_HIPU(hipMemcpy(U->d_nbr, nbr_em.data(), sizeof(int32_t) * 4 * n, hipMemcpyHostToDevice));
     HF_HIPU(hipMemcpy(U->d_len, len.data(), sizeof(double) * 4 * n, hipMemcpyHostToDevice));
-    HF_HIPU(hipMemcpy(U->d_cs, cs.data(), sizeof(double) * 16 * n, hipMemcpyHostToDevice));
+    HF_HIPU(hipMemcpy(U->d_cs, cst.data(), sizeof(double) * cst.size(), hipMemcpyHostToDevice));
     HF_HIPU(hipMemcpy(U->d_iv, iv.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-    HF_HIPU(hipMemcpy(U->d_dr, dr.data(), sizeof(double) * n, hipMemcpyHostToDevice));
     HF_HIPU(hipMemset(U->buf[0], 0, sizeof(double) * 4 * n));
     HF_HIPU(hipMemset(U->buf[1], 0, sizeof(double) * 4 * n));
     HF_HIPU(hipMemset(U->d_hist, 0, sizeof(double) * U->hist_cap));
@@ -261,7 +271,7 @@ int hf_uns_create(const hf_uns_desc* d, hf_uns** out)
     m.len = U->d_len;
     m.cs = U->d_cs;
     m.inv_vol = U->d_iv;
-    m.dr = U->d_dr;
+    m.cs_terms = terms;
     m.gamma = U->gamma;
     m.cfl = U->cfl;
     for (int k = 0; k < 4; k++) m.inlet[k] = U->inlet[k];
@@ -278,7 +288,7 @@ void hf_uns_destroy(hf_uns* u)
             hipEventDestroy(e.first);
             hipEventDestroy(e.second);
         }
-    for (void* p : {(void*)u->d_nbr, (void*)u->d_len, (void*)u->d_cs, (void*)u->d_iv, (void*)u->d_dr, (void*)u->buf[0],
+    for (void* p : {(void*)u->d_nbr, (void*)u->d_len, (void*)u->d_cs, (void*)u->d_iv, (void*)u->buf[0],
                     (void*)u->buf[1], (void*)u->d_slots, (void*)u->d_hist})
         if (p) hipFree(p);
     if (u->stream) hipStreamDestroy(u->stream);
@@ -286,6 +296,12 @@ void hf_uns_destroy(hf_uns* u)
 }
 
 int hf_uns_cells(const hf_uns* u) { return u ? u->n : 0; }
+
+int hf_uns_bytes_per_cell(const hf_uns* u)
+{
+    if (!u) return fail(HF_EINVAL, "hf_uns_bytes_per_cell: null context");
+    return 2 * 4 * 8 + 4 * 4 + 4 * 8 + 4 * u->mesh.cs_terms * 8 + 8;
+}
 
 int hf_uns_connectivity(const hf_uns* u, int32_t* out)
 {

@@ -12,9 +12,10 @@ struct UnsMesh {
     int32_t n, pad;
     const int32_t* nbr;      // [4][n] neighbour cell or boundary code
     const double* len;       // [4][n] edge length
-    const double* cs;        // [4][4][n] cos, sin of the outward normal angle, then of its negative
+    const double* cs;        // [4][cs_terms][n]: cos, sin of the outward normal angle, then (cs_terms 4)
+                             // cos, sin of its negative; with cs_terms 2 those are cos and -sin
     const double* inv_vol;   // [n] 1 / cell volume
-    const double* dr;        // [n] incircle radius (shortest edge)
+    int32_t cs_terms, pad2;
     double gamma, cfl;
     double inlet[4];
 };

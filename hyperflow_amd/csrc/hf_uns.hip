@@ -46,6 +46,17 @@ __device__ __forceinline__ unsigned long long wave_min_bits(unsigned long long b
     return b;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  The grid is padded to a multiple of 8 and workgroup b works on cell
+// chunk (b % 8) * (grid / 8) + b / 8, so each XCD sweeps one contiguous
+// eighth of the cells and finds most neighbour states in its own L2.
+constexpr int XCDS = 8;
+__device__ __forceinline__ int cell_chunk()
+{
+    return (int)(blockIdx.x % XCDS) * (int)(gridDim.x / XCDS) + (int)(blockIdx.x / XCDS);
+}
+inline dim3 chunk_grid(int n) { return dim3((unsigned)(((n + 255) / 256 + XCDS - 1) / XCDS * XCDS)); }
+
 // The workgroup's minimum into one way of a slot: wave minima through LDS,
 // then one atomic.  Called by every thread of the workgroup.
 __device__ __forceinline__ void block_min_to_slot(unsigned long long cand, unsigned long long* slot)
@@ -66,7 +77,7 @@ __device__ __forceinline__ void block_min_to_slot(unsigned long long cand, unsig
 // ForwardEulerODESolver (scheme/ode/forward_euler/forward_euler.cpp:16-30).
 __global__ __launch_bounds__(256) void k_uns_step(const UnsMesh m, const UnsStep s)
 {
-    const int c = blockIdx.x * 256 + threadIdx.x;
+    const int c = cell_chunk() * 256 + threadIdx.x;
     const int n = m.n;
     const double gamma = m.gamma, g1 = m.gamma - 1.0;
     const int lane = threadIdx.x & 63;
@@ -87,11 +98,17 @@ __global__ __launch_bounds__(256) void k_uns_step(const UnsMesh m, const UnsStep
 #pragma unroll
         for (int k = 0; k < 4; k++) U[k] = s.A[(size_t)k * n + c];
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        double shortest = 1e16;
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             const int nb = m.nbr[(size_t)e * n + c];
-            const double cr = m.cs[(size_t)(4 * e + 0) * n + c], sr = m.cs[(size_t)(4 * e + 1) * n + c];
-            const double cb = m.cs[(size_t)(4 * e + 2) * n + c], sb = m.cs[(size_t)(4 * e + 3) * n + c];
+            const int ct = m.cs_terms;
+            const double cr = m.cs[(size_t)(ct * e + 0) * n + c], sr = m.cs[(size_t)(ct * e + 1) * n + c];
+            double cb = cr, sb = -sr;   // the back rotation mirrors the forward one (checked on the host)
+            if (ct == 4) {
+                cb = m.cs[(size_t)(4 * e + 2) * n + c];
+                sb = m.cs[(size_t)(4 * e + 3) * n + c];
+            }
             double V[4];
             if (nb >= 0) {
 #pragma unroll
@@ -116,6 +133,7 @@ __global__ __launch_bounds__(256) void k_uns_step(const UnsMesh m, const UnsStep
             hf::hllc<MathF64, true>(L, Rs, F);
             rotate(cb, sb, F, f);
             const double len = m.len[(size_t)e * n + c];
+            shortest = (shortest < len) ? shortest : len;   // Polygon's incircle radius (polygon.cpp:79-91)
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 f[k] *= len;
@@ -134,7 +152,7 @@ __global__ __launch_bounds__(256) void k_uns_step(const UnsMesh m, const UnsStep
             N[k] = x;
             s.B[(size_t)k * n + c] = x;
         }
-        if (s.wr >= 0) cand = bits_of(cell_dt(N, m.dr[c], gamma, g1));
+        if (s.wr >= 0) cand = bits_of(cell_dt(N, shortest, gamma, g1));
     }
     if (s.wr >= 0) block_min_to_slot(cand, s.slots + s.wr * SLOT_WORDS);
 }
@@ -142,14 +160,20 @@ __global__ __launch_bounds__(256) void k_uns_step(const UnsMesh m, const UnsStep
 __global__ __launch_bounds__(256) void k_uns_scan(const UnsMesh m, const double* __restrict__ A,
                                                   unsigned long long* slot)
 {
-    const int c = blockIdx.x * 256 + threadIdx.x;
+    const int c = cell_chunk() * 256 + threadIdx.x;
     const int n = m.n;
     unsigned long long cand = ~0ull;
     if (c < n) {
         double U[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) U[k] = A[(size_t)k * n + c];
-        cand = bits_of(cell_dt(U, m.dr[c], m.gamma, m.gamma - 1.0));
+        double shortest = 1e16;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const double len = m.len[(size_t)e * n + c];
+            shortest = (shortest < len) ? shortest : len;
+        }
+        cand = bits_of(cell_dt(U, shortest, m.gamma, m.gamma - 1.0));
     }
     block_min_to_slot(cand, slot);
 }
@@ -158,15 +182,15 @@ __global__ void k_uns_reset(unsigned long long* slot) { slot[threadIdx.x * SLOT_
 
 hipError_t launch_uns_step(const UnsMesh& m, const UnsStep& s, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
-    if (e0 || e1) hipExtLaunchKernelGGL(k_uns_step, dim3((m.n + 255) / 256), dim3(256), 0, st, e0, e1, 0, m, s);
-    else hipLaunchKernelGGL(k_uns_step, dim3((m.n + 255) / 256), dim3(256), 0, st, m, s);
+    if (e0 || e1) hipExtLaunchKernelGGL(k_uns_step, chunk_grid(m.n), dim3(256), 0, st, e0, e1, 0, m, s);
+    else hipLaunchKernelGGL(k_uns_step, chunk_grid(m.n), dim3(256), 0, st, m, s);
     return hipGetLastError();
 }
 
 hipError_t launch_uns_scan(const UnsMesh& m, const double* A, unsigned long long* slots, int slot, hipStream_t st)
 {
     hipLaunchKernelGGL(k_uns_reset, dim3(1), dim3(SLOT_WAYS), 0, st, slots + slot * SLOT_WORDS);
-    hipLaunchKernelGGL(k_uns_scan, dim3((m.n + 255) / 256), dim3(256), 0, st, m, A, slots + slot * SLOT_WORDS);
+    hipLaunchKernelGGL(k_uns_scan, chunk_grid(m.n), dim3(256), 0, st, m, A, slots + slot * SLOT_WORDS);
     return hipGetLastError();
 }
 

@@ -258,7 +258,7 @@ class hf_uns_desc(C.Structure):
 
 EXPORTS_UNS = ("hf_uns_create", "hf_uns_destroy", "hf_uns_cells", "hf_uns_connectivity", "hf_uns_geometry",
                "hf_uns_upload", "hf_uns_download", "hf_uns_time_step", "hf_uns_advance", "hf_uns_run",
-               "hf_uns_synchronize", "hf_uns_dt_history", "hf_uns_profile", "hf_uns_kernel_stats")
+               "hf_uns_synchronize", "hf_uns_dt_history", "hf_uns_profile", "hf_uns_kernel_stats", "hf_uns_bytes_per_cell")
 
 _bound = False
 
@@ -284,6 +284,7 @@ def _lib():
         L.hf_uns_synchronize.argtypes = [vp]
         L.hf_uns_dt_history.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int)]
         L.hf_uns_profile.argtypes = [vp, C.c_int]
+        L.hf_uns_bytes_per_cell.argtypes = [vp]
         L.hf_uns_kernel_stats.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
         _bound = True
     return L
@@ -368,6 +369,12 @@ class UnsSolver:
         n, ms = C.c_int64(), C.c_double()
         _check(_lib().hf_uns_kernel_stats(self._h, C.byref(n), C.byref(ms)), "hf_uns_kernel_stats")
         return n.value, ms.value
+
+    def bytes_per_cell(self) -> int:
+        """Bytes a step reads and writes per cell (each array once)."""
+        b = _lib().hf_uns_bytes_per_cell(self._h)
+        _check(0 if b > 0 else b, "hf_uns_bytes_per_cell")
+        return b
 
     def dt_history(self) -> np.ndarray:
         n = C.c_int()

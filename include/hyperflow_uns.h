@@ -67,6 +67,11 @@ HF_API int hf_uns_dt_history(hf_uns* u, double* out, int max, int* count);
  * step launch while enabled; stats return launches and summed milliseconds
  * since the last call, then reset. */
 HF_API int hf_uns_profile(hf_uns* u, int enable);
+/* Bytes one step reads and writes per cell, counting each array once:
+ * state in and out, neighbour indices, edge lengths, rotation terms (2 per
+ * edge when the back rotation mirrors the forward one on every edge, else
+ * 4), 1/V.  No counterpart in the reference (roofline accounting). */
+HF_API int hf_uns_bytes_per_cell(const hf_uns* u);
 HF_API int hf_uns_kernel_stats(hf_uns* u, int64_t* launches, double* total_ms);
 
 #ifdef __cplusplus
