@@ -154,6 +154,19 @@ def uns_cases():
                             {"type": "SphericalRiemannProblem", "radius": 0.3, "x_origin": 0.5, "y_origin": 0.5,
                              "variables": "Primitive", "init_sphere_state": [1.0, 0.5, -0.25, 2.0],
                              "init_ext_state": [0.5, 0.0, 0.0, 0.4]}, cfl=0.5), 30))
+    # the loader's quirks: a boundary code on an edge that has a neighbour (the
+    # neighbour wins) and an interior code on an outer edge (no neighbour: the
+    # edge points at cell 0)
+    q, c = U.box(5, 4, code=U.TRANSMISSIVE, skew=0.2)
+    c[6, 1] = U.REFLECTIVE       # interior edge between cells 6 and 7
+    c[12, 2] = U.INLET           # interior edge between cells 12 and 17
+    c[9, 1] = 0                  # outer right edge of cell 9
+    c[15, 3] = 0                 # outer left edge of cell 15
+    out.append(("uns_loader_quirks_5x4", q, c,
+                U.case_json("", (0.0, 1.0, 0.0, 1.0), [1.2, 0.4, 0.1, 1.1],
+                            {"type": "SphericalRiemannProblem", "radius": 0.3, "x_origin": 0.5, "y_origin": 0.5,
+                             "variables": "Primitive", "init_sphere_state": [1.0, 0.2, -0.1, 2.0],
+                             "init_ext_state": [0.5, 0.0, 0.0, 0.5]}, cfl=0.5), 20))
     return out
 
 
