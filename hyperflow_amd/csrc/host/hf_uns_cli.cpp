@@ -185,7 +185,7 @@ int main(int argc, char* argv[])
             check(hf_uns_download(u, f.data(), f.size()));
             write_frame(outdir, frame_idx, tt, cfg, model, cen, f, n);
         };
-        for (; n_step < 100000000u; n_step++) {
+        for (; n_step < Simulation::MAX_STEPS; n_step++) {   // simulation.h:50
             if (!quiet) {
                 const double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
                 std::cout << "Steps: " << n_step << ", T: " << t << ", dt: " << dt << ", complete: "
