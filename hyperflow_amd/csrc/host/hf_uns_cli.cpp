@@ -185,7 +185,8 @@ int main(int argc, char* argv[])
             check(hf_uns_download(u, f.data(), f.size()));
             write_frame(outdir, frame_idx, tt, cfg, model, cen, f, n);
         };
-        for (; n_step < Simulation::MAX_STEPS; n_step++) {   // simulation.h:50
+        const unsigned max_steps = 1000000000u;   // Simulation::MAX_STEPS (simulation.h:50)
+        for (; n_step < max_steps; n_step++) {
             if (!quiet) {
                 const double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
                 std::cout << "Steps: " << n_step << ", T: " << t << ", dt: " << dt << ", complete: "
