@@ -260,12 +260,14 @@ def save(net, path: str, meta: dict) -> None:
              max_output=np.float32(d["max_output"]), meta=np.array(json.dumps(meta)))
 
 
-def load(path: str = WEIGHTS, shock_threshold=None) -> native.Net:
+def load(path: str = WEIGHTS, shock_threshold=None, max_output=None) -> native.Net:
+    """The network saved at `path`; shock_threshold / max_output override the
+    stored fallback settings."""
     z = np.load(path, allow_pickle=False)
     th = float(z["shock_threshold"]) if shock_threshold is None else shock_threshold
+    mo = (float(z["max_output"]) if "max_output" in z.files else 0.0) if max_output is None else max_output
     net = native.Net(in_mean=z["in_mean"], in_scale=z["in_scale"], out_scale=z["out_scale"],
-                     gamma=float(z["gamma"]), shock_threshold=th,
-                     max_output=float(z["max_output"]) if "max_output" in z.files else 0.0)
+                     gamma=float(z["gamma"]), shock_threshold=th, max_output=mo)
     net.set_params(z["params"])
     return net
 
@@ -469,6 +471,7 @@ def main(argv=None):
     e.add_argument("--every", type=int, nargs="+", default=None, help="default: the interval stored with the weights")
     e.add_argument("--start", type=int, default=None, help="first corrected step; default: stored with the weights")
     e.add_argument("--shock-threshold", type=float, default=None)
+    e.add_argument("--max-output", type=float, default=None, help="fallback bound on |network output|")
     e.add_argument("--resolutions", type=int, nargs="+", default=[1024, 1280, 1536, 1792])
     e.add_argument("--json", default=None)
     args = ap.parse_args(argv)
@@ -515,7 +518,7 @@ def main(argv=None):
         save(net, args.out, meta)
         return 0
     ensure_tuned(sorted(set((COARSE, FINE) + tuple(args.resolutions))), log=log)
-    net = load(args.weights, args.shock_threshold)
+    net = load(args.weights, args.shock_threshold, args.max_output)
     meta = load_meta(args.weights)
     every = args.every or [int(meta.get("every", 1))]
     start = args.start if args.start is not None else int(meta.get("start", 0))
