@@ -374,8 +374,8 @@ int hf_uns_advance(hf_uns* u, double dt)
 
 int hf_uns_run(hf_uns* u, int nsteps, int flags)
 {
-    (void)flags;
     if (!u) return fail(HF_EINVAL, "hf_uns_run: null context");
+    if (flags != 0) return fail(HF_EINVAL, "hf_uns_run: flags must be 0");
     if (nsteps < 0) return fail(HF_EINVAL, "hf_uns_run: nsteps must be >= 0");
     if (nsteps == 0) return HF_OK;
     int rc = ensure_slot(u);

@@ -58,7 +58,8 @@ HF_API int hf_uns_time_step(hf_uns* u, double* dt);
  * (godunov.cpp:31-77, forward_euler.cpp:16-30). */
 HF_API int hf_uns_advance(hf_uns* u, double dt);
 /* nsteps of Simulation::solve's step body (simulation.cpp:85-118, without
- * clamps or output) with the dt kept on the device; asynchronous. */
+ * clamps or output) with the dt kept on the device; asynchronous.  One launch
+ * per step; flags must be 0 (no graph mode: a step is a single launch). */
 HF_API int hf_uns_run(hf_uns* u, int nsteps, int flags);
 HF_API int hf_uns_synchronize(hf_uns* u);
 /* Every TimeStep result so far, in call order. */

@@ -143,6 +143,8 @@ def test_bad_sizes_are_rejected():
         U._check(U._lib().hf_uns_upload(s._h, np.zeros(5), 5), "hf_uns_upload")
     with pytest.raises(Exception, match="nsteps"):
         U._check(U._lib().hf_uns_run(s._h, -1, 0), "hf_uns_run")
+    with pytest.raises(Exception, match="flags"):
+        U._check(U._lib().hf_uns_run(s._h, 1, 1), "hf_uns_run")
     s.close()
 
 
