@@ -382,8 +382,9 @@ __global__ __launch_bounds__(256) void k_godunov(const HfStepArgs<typename M::R>
 // ghosts' max wave speeds per block, and (step mode) advances each problem's
 // TimeStep call counter and clears the partial maxima of the next step.
 
+constexpr int GHOST_WAVES = 16;   // k_ghost workgroups: 1024 threads
 template <class M>
-__global__ __launch_bounds__(256) void k_ghost(const HfGhost* __restrict__ tab, int n,
+__global__ __launch_bounds__(1024) void k_ghost(const HfGhost* __restrict__ tab, int n,
                                                const HfBlockDev* __restrict__ blocks,
                                                const typename M::R* __restrict__ S, typename M::R* __restrict__ D,
                                                const typename M::R* __restrict__ consts, unsigned long long* gmax,
@@ -438,19 +439,52 @@ __global__ __launch_bounds__(256) void k_ghost(const HfGhost* __restrict__ tab, 
         dy_ = dmax_nan(0.0, y);
         blk = e.dst_block;
     }
-    // one atomic per wave when the wave covers a single block (the table is
-    // grouped by destination block), per lane otherwise
+    // The table is grouped by destination block: a wave covering one block
+    // reduces to one value, and thread 0 merges consecutive waves of the same
+    // block, so a workgroup issues one atomic per block it covers (device-
+    // scope atomics on one address serialise; one per wave made this launch
+    // 5x longer).  A wave spanning two blocks uses per-lane atomics.
+    __shared__ int wblk[GHOST_WAVES];
+    __shared__ double wmx[GHOST_WAVES], wmy[GHOST_WAVES];
+    const int w = threadIdx.x >> 6;
     const int b0 = __shfl(blk, 0, 64);
     const bool uniform = __all(blk == b0 || blk < 0);
     if (uniform) {
         const double wx = wave_max(dx_), wy = wave_max(dy_);
-        if (lane == 0 && b0 >= 0) {
-            atomicMax(&gmax[2 * b0], dbits(wx));
-            atomicMax(&gmax[2 * b0 + 1], dbits(wy));
+        if (lane == 0) {
+            wblk[w] = b0;
+            wmx[w] = wx;
+            wmy[w] = wy;
         }
-    } else if (blk >= 0) {
-        atomicMax(&gmax[2 * blk], dbits(dx_));
-        atomicMax(&gmax[2 * blk + 1], dbits(dy_));
+    } else {
+        if (blk >= 0) {
+            atomicMax(&gmax[2 * blk], dbits(dx_));
+            atomicMax(&gmax[2 * blk + 1], dbits(dy_));
+        }
+        if (lane == 0) wblk[w] = -1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int cb = -1;
+        double cx = 0.0, cy = 0.0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); k++) {
+            if (wblk[k] == cb) {
+                cx = (wmx[k] > cx) ? wmx[k] : cx;
+                cy = (wmy[k] > cy) ? wmy[k] : cy;
+                continue;
+            }
+            if (cb >= 0) {
+                atomicMax(&gmax[2 * cb], dbits(cx));
+                atomicMax(&gmax[2 * cb + 1], dbits(cy));
+            }
+            cb = wblk[k];
+            cx = wmx[k];
+            cy = wmy[k];
+        }
+        if (cb >= 0) {
+            atomicMax(&gmax[2 * cb], dbits(cx));
+            atomicMax(&gmax[2 * cb + 1], dbits(cy));
+        }
     }
 }
 

@@ -59,8 +59,9 @@ hipError_t launch_ghost<Real>(const HfGhost* tab, int n, const HfBlockDev* block
                               const Real* consts, unsigned long long* gmax, double gamma, const GhostBook& book,
                               hipStream_t s)
 {
-    const int nb = n > 0 ? (n + 255) / 256 : 1;
-    hipLaunchKernelGGL((k_ghost<Math>), dim3(nb), dim3(256), 0, s, tab, n, blocks, S, D, consts, gmax, gamma, book);
+    const int nb = n > 0 ? (n + 64 * GHOST_WAVES - 1) / (64 * GHOST_WAVES) : 1;
+    hipLaunchKernelGGL((k_ghost<Math>), dim3(nb), dim3(64 * GHOST_WAVES), 0, s, tab, n, blocks, S, D, consts, gmax,
+                       gamma, book);
     return hipGetLastError();
 }
 
