@@ -40,7 +40,6 @@ namespace {
 inline int fail(int code, const std::string& msg) { return hf_fail(code, msg); }
 
 inline double smin(double a, double b) { return (b < a) ? b : a; }
-inline double smax(double a, double b) { return (a < b) ? b : a; }
 inline double bitsd(unsigned long long b)
 {
     double d;
@@ -488,8 +487,8 @@ int download_block(hf_ctx* c, int b, const void* base, int64_t off, double* out)
 void destroy_graphs(hf_ctx* c)
 {
     for (auto& kv : c->graphs) {
-        hipGraphExecDestroy(kv.second.second);
-        hipGraphDestroy(kv.second.first);
+        (void)hipGraphExecDestroy(kv.second.second);
+        (void)hipGraphDestroy(kv.second.first);
     }
     c->graphs.clear();
 }
@@ -684,18 +683,18 @@ int hf_create(const hf_case_desc* desc, hf_ctx** out)
 void hf_destroy(hf_ctx* c)
 {
     if (!c) return;
-    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     destroy_graphs(c);
-    for (auto& e : c->ev_pending) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
-    for (auto& e : c->ev_free) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
+    for (auto& e : c->ev_pending) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    for (auto& e : c->ev_free) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
     for (void* p : {c->buf[0], c->buf[1], (void*)c->d_blocks, (void*)c->d_probs, (void*)c->d_items, (void*)c->d_fill,
                     (void*)c->d_copy, c->d_consts, (void*)c->d_imax, (void*)c->d_gmax, (void*)c->d_scan,
                     (void*)c->d_calls, (void*)c->d_hist, (void*)c->d_part, c->d_resid, (void*)c->d_imax_fix,
                     (void*)c->d_fixed, (void*)c->d_queue, (void*)c->d_skip, (void*)c->d_skip_next})
-        if (p) hipFree(p);
-    if (c->h_calls_pinned) hipHostFree(c->h_calls_pinned);
-    if (c->calls_copied) hipEventDestroy(c->calls_copied);
-    if (c->stream) hipStreamDestroy(c->stream);
+        if (p) (void)hipFree(p);
+    if (c->h_calls_pinned) (void)hipHostFree(c->h_calls_pinned);
+    if (c->calls_copied) (void)hipEventDestroy(c->calls_copied);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -832,7 +831,7 @@ static int run_loop(hf_ctx* c, int nsteps, int flags, bool fixed, double dt)
             rc = enqueue_steps(c, nsteps, false, fixed, dt);
             hipError_t ce = hipStreamEndCapture(c->stream, &g);
             if (rc) {
-                if (g) hipGraphDestroy(g);
+                if (g) (void)hipGraphDestroy(g);
                 return rc;
             }
             HF_HIP(ce);

@@ -77,15 +77,15 @@ namespace {
 
 void free_images(hf_net* n)
 {
-    hipFree(n->x);
-    for (auto& p : n->act) hipFree(p), p = nullptr;
-    hipFree(n->g8);
-    for (auto& p : n->gz) hipFree(p), p = nullptr;
-    hipFree(n->y);
-    hipFree(n->shock);
-    hipFree(n->part);
-    hipFree(n->part2);
-    hipFree(n->lossp);
+    (void)hipFree(n->x);
+    for (auto& p : n->act) (void)hipFree(p), p = nullptr;
+    (void)hipFree(n->g8);
+    for (auto& p : n->gz) (void)hipFree(p), p = nullptr;
+    (void)hipFree(n->y);
+    (void)hipFree(n->shock);
+    (void)hipFree(n->part);
+    (void)hipFree(n->part2);
+    (void)hipFree(n->lossp);
     n->x = n->g8 = nullptr;
     n->y = n->part = n->part2 = n->lossp = nullptr;
     n->shock = nullptr;
@@ -309,19 +309,19 @@ int hf_net_create(const hf_net_desc* desc, hf_net** out)
 void hf_net_destroy(hf_net* n)
 {
     if (!n) return;
-    hipDeviceSynchronize();
+    (void)hipDeviceSynchronize();
     free_images(n);
-    hipFree(n->params);
-    hipFree(n->grads);
-    hipFree(n->m);
-    hipFree(n->v);
-    hipFree(n->loss);
-    hipFree(n->fallback);
+    (void)hipFree(n->params);
+    (void)hipFree(n->grads);
+    (void)hipFree(n->m);
+    (void)hipFree(n->v);
+    (void)hipFree(n->loss);
+    (void)hipFree(n->fallback);
     for (int l = 0; l < NL; l++) {
-        hipFree(n->wf[l]);
-        hipFree(n->wb[l]);
+        (void)hipFree(n->wf[l]);
+        (void)hipFree(n->wb[l]);
     }
-    if (n->done) hipEventDestroy(n->done);
+    if (n->done) (void)hipEventDestroy(n->done);
     delete n;
 }
 

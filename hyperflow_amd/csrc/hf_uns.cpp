@@ -282,16 +282,16 @@ int hf_uns_create(const hf_uns_desc* d, hf_uns** out)
 void hf_uns_destroy(hf_uns* u)
 {
     if (!u) return;
-    if (u->stream) hipStreamSynchronize(u->stream);
+    if (u->stream) (void)hipStreamSynchronize(u->stream);
     for (auto* list : {&u->ev_pending, &u->ev_free})
         for (auto& e : *list) {
-            hipEventDestroy(e.first);
-            hipEventDestroy(e.second);
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
         }
     for (void* p : {(void*)u->d_nbr, (void*)u->d_len, (void*)u->d_cs, (void*)u->d_iv, (void*)u->buf[0],
                     (void*)u->buf[1], (void*)u->d_slots, (void*)u->d_hist})
-        if (p) hipFree(p);
-    if (u->stream) hipStreamDestroy(u->stream);
+        if (p) (void)hipFree(p);
+    if (u->stream) (void)hipStreamDestroy(u->stream);
     delete u;
 }
 
