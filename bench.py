@@ -321,7 +321,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if precision == "fp64" else "f32",
-        "data": "synthetic (seeded initial states of the named workload)",
+        "data": ("synthetic (the channel filled with its Mach 3 inflow state, as in the reference case)"
+                 if args.workload == "c2" else "synthetic (seeded quadrant states of the named workload)"),
         "config": {"workload": workload_text, "blocks": sum(len(c.blocks) for c in cases),
                    "interior_cells": cells, "scheme": cases[0].scheme, "limiter": cases[0].limiter,
                    "riemann": "HLLC", "cfl": cases[0].cfl, "tile_rows": tile_rows or "default",
