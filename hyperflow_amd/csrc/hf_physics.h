@@ -263,6 +263,16 @@ __device__ __forceinline__ double wave_max(double x)
     return x;
 }
 
+// Reduce a per-lane minimum over the wave (NaN-free inputs).
+__device__ __forceinline__ double wave_min(double x)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(x, off, 64);
+        x = (o < x) ? o : x;
+    }
+    return x;
+}
+
 // The step and predictor dt of a block from the device partial maxima
 // (TimeStep::operator(), timestep.cpp:29-68).  For a block the minimum of
 // dx/(|u|+c) over its cells equals dx / max(|u|+c): correctly rounded
@@ -298,12 +308,18 @@ __device__ __forceinline__ void step_dts(const HfStepArgs<R>& a, int block, cons
                                          int view, double& dt_step, double& dt_pred, long long& kstep,
                                          long long& kpred)
 {
+    // Lane l takes blocks first+l, first+l+64, ...: the per-block loads and
+    // divisions run side by side instead of as one dependent chain, then the
+    // minima are reduced over the wave.  A minimum of non-NaN values does not
+    // depend on the order, so the results equal the sequential loop's.  Every
+    // lane of the wave calls this (wave-uniform arguments).
     const HfProbDev pr = a.probs[bd.problem];
     kstep = (long long)a.calls[bd.problem] - pr.cps + 1;
     kpred = kstep + 1 + bd.lb;
     double dtx = 1e16, dty = 1e16, ptx = 1e16, pty = 1e16;
     const bool compat = muscl && a.dt_mode == 0;
-    for (int c = pr.first; c < pr.first + pr.nb; c++) {
+    const int end = pr.first + pr.nb;
+    for (int c = pr.first + (int)(threadIdx.x & 63); c < end; c += 64) {
         const double cdx = a.blocks[c].dx, cdy = a.blocks[c].dy;
         const double gx = bitsd(a.gmax[2 * c]), gy = bitsd(a.gmax[2 * c + 1]);
         const double ox = smax(bitsd(a.imax_old[2 * c]), gx), oy = smax(bitsd(a.imax_old[2 * c + 1]), gy);
@@ -320,8 +336,16 @@ __device__ __forceinline__ void step_dts(const HfStepArgs<R>& a, int block, cons
             }
         }
     }
+    dtx = wave_min(dtx);
+    dty = wave_min(dty);
     dt_step = cfl_eff(a.cfl, kstep) * smin(dtx, dty);
-    dt_pred = compat ? cfl_eff(a.cfl, kpred) * smin(ptx, pty) : dt_step;
+    if (compat) {
+        ptx = wave_min(ptx);
+        pty = wave_min(pty);
+        dt_pred = cfl_eff(a.cfl, kpred) * smin(ptx, pty);
+    } else {
+        dt_pred = dt_step;
+    }
 }
 
 }  // namespace hf
