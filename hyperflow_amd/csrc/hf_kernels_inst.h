@@ -59,9 +59,7 @@ hipError_t launch_ghost<Real>(const HfGhost* tab, int n, const HfBlockDev* block
                               const Real* consts, unsigned long long* gmax, double gamma, const GhostBook& book,
                               hipStream_t s)
 {
-    // large fills: 1024-thread workgroups (fewer maxima atomics); small ones:
-    // 256 threads, so the few waves do not wait on each other
-    const int threads = n > 16384 ? 64 * GHOST_WAVES : 256;
+    const int threads = 64 * GHOST_WAVES;
     const int nb = n > 0 ? (n + threads - 1) / threads : 1;
     hipLaunchKernelGGL((k_ghost<Math>), dim3(nb), dim3(threads), 0, s, tab, n, blocks, S, D, consts, gmax, gamma,
                        book);
