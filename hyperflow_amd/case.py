@@ -328,6 +328,31 @@ def perturbed_fields(case: Case, seed: int) -> List[np.ndarray]:
     return fields
 
 
+def nonphysical_fields(case: Case, seed: int, block: int = 0) -> List[np.ndarray]:
+    """perturbed_fields with three interior cells of one block made
+    non-physical: negative total energy (so negative pressure and a NaN sound
+    speed), zero density under non-zero momentum, and a NaN y-momentum.  They
+    exercise the NaN paths of the scheme: HLLC's zero flux for NaN wave
+    speeds, the limiters' comparisons with NaN ratios, and TimeStep skipping
+    NaN terms."""
+    fields = perturbed_fields(case, seed)
+    g = case.ghost
+    b = case.blocks[block]
+    U = fields[block]
+
+    def cell(fx: float, fy: float):
+        return g + int(fy * b.ny), g + int(fx * b.nx)
+
+    y, x = cell(0.3, 0.25)
+    U[3, y, x] = -0.5 * abs(U[3, y, x])
+    y, x = cell(0.55, 0.6)
+    U[0, y, x] = 0.0
+    U[1, y, x] = 0.3
+    y, x = cell(0.2, 0.8)
+    U[2, y, x] = np.nan
+    return fields
+
+
 def config3(seed: int = 0, n: int = 8192, limiter: str = "VanLeer") -> Case:
     """MUSCL-Hancock + Van Leer, one n x n block, transmissive, CFL 0.5
     (fields come from perturbed_fields)."""

@@ -8,6 +8,7 @@ Each fixture is an .npz with the case JSON (as text), the initial padded
 fields, the fields after `nsteps` steps and every TimeStep result.  Only data
 is stored; no reference source.
 
+    python tests/golden/make_golden.py --nonphysical  # cells with negative pressure, zero density, NaN
     python tests/golden/make_golden.py --cli     # CSV frames of the reference program
     python tests/golden/make_golden.py --uns     # unstructured solver, small meshes
     python tests/golden/make_golden.py --kumar   # shipped scramjet case, coarse mesh
@@ -72,10 +73,23 @@ def cases():
     return out
 
 
-def main():
+def nonphysical_cases():
+    """Seeded cases with three non-physical cells (C.nonphysical_fields):
+    negative pressure, zero density, a NaN momentum."""
+    out = []
+    for name, scheme, lim, seed in (("godunov", "Godunov", None, 4), ("minbee", "MUSCLHancock", "MinBee", 5),
+                                    ("vanleer", "MUSCLHancock", "VanLeer", 6)):
+        c = C.quad_case(20, 16, seed, scheme=scheme, limiter=lim, cfl=0.8 if lim is None else 0.5)
+        out.append((f"nonphysical_{name}_20x16_s{seed}", c, C.nonphysical_fields(c, seed), (1, 5, 30)))
+    c = C.step_channel(8, "MinBee", 0.5)
+    out.append(("nonphysical_step_channel_minbee_8_s7", c, C.nonphysical_fields(c, 7, block=3), (1, 5, 30)))
+    return out
+
+
+def main(which=cases):
     if not hfo.have_reference():
         sys.exit("oracle/_ref/ref_harness is not built (needs the reference checkout)")
-    for name, case, fields, steps in cases():
+    for name, case, fields, steps in which():
         init = fields if fields is not None else case.initial_fields()
         data = {"case_json": np.array(json.dumps(case.to_json())), "nblocks": np.array(len(case.blocks))}
         for b, U in enumerate(init):
@@ -93,6 +107,8 @@ def main():
 
 if __name__ == "__main__" and len(sys.argv) == 1:
     main()
+if __name__ == "__main__" and sys.argv[1:] == ["--nonphysical"]:
+    main(nonphysical_cases)
 
 
 def cli_goldens():
