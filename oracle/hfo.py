@@ -307,10 +307,11 @@ def have_uns_reference() -> bool:
     return os.path.exists(REF_UNS_HARNESS)
 
 
-def run_uns_reference(case_json: dict, nsteps: int, timeout: float = 3600):
+def run_uns_reference(case_json: dict, nsteps: int, timeout: float = 3600, fields=None):
     """Run the reference's unstructured solver (oracle/_ref/ref_uns_harness)
-    on a case dict whose mesh.file is an absolute .hmsh path.  Returns a dict
-    with connectivity [n][4], centroids, volumes, fields [n][4], dts, elapsed."""
+    on a case dict whose mesh.file is an absolute .hmsh path, from the case's
+    initial condition or from `fields` [n][4].  Returns a dict with
+    connectivity [n][4], centroids, volumes, fields [n][4], dts, elapsed."""
     import json
     if not have_uns_reference():
         raise RuntimeError("reference unstructured harness not built")
@@ -319,8 +320,11 @@ def run_uns_reference(case_json: dict, nsteps: int, timeout: float = 3600):
         out = os.path.join(td, "out.bin")
         with open(cj, "w") as f:
             json.dump(case_json, f)
-        subprocess.run([REF_UNS_HARNESS, cj, str(nsteps), out], check=True, stdout=subprocess.DEVNULL,
-                       timeout=timeout)
+        args = [REF_UNS_HARNESS, cj, str(nsteps), out]
+        if fields is not None:
+            args.append(os.path.join(td, "fields.bin"))
+            np.ascontiguousarray(fields, np.float64).tofile(args[-1])
+        subprocess.run(args, check=True, stdout=subprocess.DEVNULL, timeout=timeout)
         data = open(out, "rb").read()
     nc, ns = struct.unpack_from("<ii", data, 0)
     off = 8

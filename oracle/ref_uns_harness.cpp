@@ -2,13 +2,15 @@
 // (src/fvm_2d_uns) step by step, for the oracle's golden vectors.  Test
 // infrastructure only; built by oracle/Makefile into oracle/_ref/.
 //
-//   ref_uns_harness case.json nsteps out.bin
+//   ref_uns_harness case.json nsteps out.bin [fields.bin]
 //
 // The case is the reference's own JSON format ("mesh.file" names the .hmsh
 // mesh).  The objects are made by the reference's JSONParser factories and
 // stepped like Simulation::solve (src/fvm_2d_uns/simulation/simulation/
 // simulation.cpp:85-118) without the t_end / frame clamps or output.  The
-// mesh loader's progress lines go to /dev/null.
+// mesh loader's progress lines go to /dev/null.  With fields.bin (f64
+// [ncells][4], conserved) the cells start from those values instead of the
+// initial condition.
 //
 // out.bin: int32 ncells, int32 nsteps;
 //          int32 [ncells*4][3] cell-edge records (cell, neighbour or boundary code, edge);
@@ -31,7 +33,7 @@ using namespace HyperFlow;
 int main(int argc, char* argv[])
 {
     if (argc < 4) {
-        std::fprintf(stderr, "usage: %s case.json nsteps out.bin\n", argv[0]);
+        std::fprintf(stderr, "usage: %s case.json nsteps out.bin [fields.bin]\n", argv[0]);
         return 2;
     }
     std::ifstream jf(argv[1]);
@@ -52,6 +54,18 @@ int main(int argc, char* argv[])
     std::shared_ptr<Scheme> scheme = parser.generate_spatial_scheme(model, riemann, time_step, mesh);
     std::shared_ptr<ODESolver> ode = parser.generate_ode_solver();
     mesh->initialise_field_values(initcon);
+    if (argc > 4) {
+        std::ifstream ff(argv[4], std::ios::binary);
+        Vec2D start(mesh->get_cells().size(), Vec1D(4));
+        for (auto& u : start)
+            for (int k = 0; k < 4; k++) ff.read(reinterpret_cast<char*>(&u[k]), 8);
+        if (!ff) {
+            std::cout.rdbuf(saved);
+            std::fprintf(stderr, "%s: short fields file\n", argv[4]);
+            return 1;
+        }
+        mesh->set_cells_flow_values(start);
+    }
 
     std::vector<double> dts;
     const auto t0 = std::chrono::steady_clock::now();

@@ -11,6 +11,7 @@ is stored; no reference source.
     python tests/golden/make_golden.py --nonphysical  # cells with negative pressure, zero density, NaN
     python tests/golden/make_golden.py --cli     # CSV frames of the reference program
     python tests/golden/make_golden.py --uns     # unstructured solver, small meshes
+    python tests/golden/make_golden.py --uns-nonphysical  # the same with non-physical cells
     python tests/golden/make_golden.py --kumar   # shipped scramjet case, coarse mesh
     python tests/golden/make_golden.py --uns-cli # CSV frames of the reference's unstructured program
 """
@@ -186,20 +187,41 @@ def uns_cases():
     return out
 
 
-def uns_goldens():
+def uns_nonphysical_cases():
+    """The ramp channel started from its inflow state with three non-physical
+    cells: negative total energy, zero density under momentum, NaN y-momentum."""
+    from hyperflow_amd import uns as U
+    q, c = U.ramp_channel(30, 12)
+    inflow = [1.4, 3.0, 0.0, 1.0]
+
+    def inject(f):
+        f = f.copy()
+        f[95, 3] = -0.5 * abs(f[95, 3])
+        f[200, 0], f[200, 1] = 0.0, 0.3
+        f[301, 2] = np.nan
+        return f
+    return [("uns_nonphysical_ramp_channel_30x12", q, c,
+             U.case_json("", (0.0, 3.0, 0.0, 1.0), inflow,
+                         {"type": "Constant", "variables": "Primitive", "init_state": inflow}, cfl=0.5), 30, inject)]
+
+
+def uns_goldens(which=None):
     """Goldens of the reference's unstructured solver
     (oracle/_ref/ref_uns_harness): full arrays for small meshes."""
     import tempfile
     from hyperflow_amd import uns as U
     if not hfo.have_uns_reference():
         sys.exit("oracle/_ref/ref_uns_harness is not built (needs the reference checkout)")
-    for name, q, c, cj, n in uns_cases():
+    for name, q, c, cj, n, *inject in (which or uns_cases)():
         with tempfile.TemporaryDirectory() as td:
             mf = os.path.join(td, "mesh.hmsh")
             U.write_hmsh(mf, q, c)
             cj["mesh"]["file"] = mf
             r0 = hfo.run_uns_reference(cj, 0)
-            r = hfo.run_uns_reference(cj, n)
+            start = inject[0](r0["fields"]) if inject else None
+            if inject:
+                r0["fields"] = start
+            r = hfo.run_uns_reference(cj, n, fields=start)
         cj["mesh"]["file"] = "mesh.hmsh"
         np.savez_compressed(os.path.join(HERE, name + ".npz"), case_json=np.array(json.dumps(cj)), quads=q,
                             codes=c, connectivity=r["connectivity"], centroids=r["centroids"],
@@ -287,6 +309,8 @@ def uns_cli_goldens():
 
 if __name__ == "__main__" and "--uns-cli" in sys.argv:
     uns_cli_goldens()
+if __name__ == "__main__" and "--uns-nonphysical" in sys.argv:
+    uns_goldens(uns_nonphysical_cases)
 if __name__ == "__main__" and "--uns" in sys.argv:
     uns_goldens()
 if __name__ == "__main__" and "--kumar" in sys.argv:

@@ -46,6 +46,11 @@ def test_oracle_matches_reference(path):
     np.testing.assert_array_equal(cen, g["centroids"])
     np.testing.assert_array_equal(vol, g["volumes"])
     init = U.initial_fields(g["quads"], cj["initial_condition"], cj["model"]["gamma"])
+    if "nonphysical" in os.path.basename(path):
+        # three cells replaced after the initial condition (make_golden.py)
+        same = (init == g["init"]) | (np.isnan(init) & np.isnan(g["init"]))
+        assert int(np.sum(~np.all(same, axis=1))) == 3
+        init = g["init"]
     np.testing.assert_array_equal(init, g["init"])
     o.set_fields(init)
     dts = o.run(int(g["steps"]))
