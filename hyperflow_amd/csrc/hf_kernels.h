@@ -70,12 +70,24 @@ __device__ __forceinline__ unsigned next_ticket(uint32_t* queue, int lane)
     return __builtin_amdgcn_readfirstlane(t) + total_waves();
 }
 
+// A wave-reduced value as a wave-uniform (scalar) bit pattern.
+__device__ __forceinline__ unsigned long long uniform_bits(double x)
+{
+    const unsigned long long b = dbits(x);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+
 // Per-wave maxima into the block's slots; every lane holds the reduced value,
-// so every lane may issue the (idempotent) atomic -- no divergent region.
+// so every lane may issue the (idempotent) atomic -- no divergent region.  The
+// value is passed as a scalar: the compiler then issues one atomic from one
+// lane, instead of first reducing 64 lane values it cannot tell are equal (a
+// 64-iteration scalar loop per atomic).
 __device__ __forceinline__ void store_maxima(unsigned long long* out, int block, double mx, double my)
 {
-    atomicMax(&out[2 * block], dbits(mx));
-    atomicMax(&out[2 * block + 1], dbits(my));
+    atomicMax(&out[2 * block], uniform_bits(mx));
+    atomicMax(&out[2 * block + 1], uniform_bits(my));
 }
 
 // Load row y of this lane's column: ghost rows/columns come from the ghost
