@@ -212,43 +212,33 @@ __device__ __forceinline__ void muscl_item(const HfStepArgs<typename M::R>& a, c
 // was skipped or its speculative dt differs, so results equal the
 // reference's sequential sweep bit for bit.  The fix-up launch also records
 // whether the block will need it again (skip_next).
-template <class R>
-__device__ __forceinline__ bool needs_fix(const HfStepArgs<R>& a, int b, const HfBlockDev& bd, double& exact)
-{
-    double s0, spec;
-    long long k0, k1;
-    step_dts<R>(a, b, bd, true, DT_SPEC, s0, spec, k0, k1);
-    step_dts<R>(a, b, bd, true, DT_FINAL, s0, exact, k0, k1);
-    return a.skip[b] || spec != exact;
-}
-
 template <class M, int LIM, bool RESID>
 __global__ __launch_bounds__(256) void k_muscl(const HfStepArgs<typename M::R> a)
 {
     typedef typename M::R R;
     const int lane = threadIdx.x & 63;
     const bool fix = a.fix_lb > 0;
+    int cached = -1;          // block whose dts (and fix verdict) this wave has computed
+    double c_step = 0.0, c_pred = 0.0;
+    long long c_kstep = 0, c_kpred = 0;
+    bool c_need = false;
     if (fix) {
-        // leave at once unless some problem's fix_lb-th block needs computing
+        // leave at once unless some problem's fix_lb-th block needs computing;
+        // the last block examined stays cached for the item loop
         bool any = false;
         for (int p = 0; p < a.nprob; p++) {
             const HfProbDev pr = a.probs[p];
             if (pr.nb <= a.fix_lb) continue;
             const int b = pr.first + a.fix_lb;
-            const HfBlockDev bd = a.blocks[b];
-            double s0, spec, exact;
-            long long k0, k1;
-            step_dts<R>(a, b, bd, true, DT_SPEC, s0, spec, k0, k1);
-            step_dts<R>(a, b, bd, true, DT_FINAL, s0, exact, k0, k1);
-            a.skip_next[b] = spec != exact ? 1u : 0u;    // same value from every wave
-            any = any || a.skip[b] || spec != exact;
+            double spec;
+            step_dts_fix<R>(a, p, pr, a.fix_lb, c_step, spec, c_pred, c_kstep, c_kpred);
+            a.skip_next[b] = spec != c_pred ? 1u : 0u;   // same value from every wave
+            c_need = a.skip[b] || spec != c_pred;
+            cached = b;
+            any = any || c_need;
         }
         if (!any) return;
     }
-    int cached = -1;          // block whose dts (and fix verdict) this wave has computed
-    double c_step = 0.0, c_pred = 0.0;
-    long long c_kstep = 0, c_kpred = 0;
-    bool c_need = false;
     for (unsigned idx = first_ticket();; idx = next_ticket(a.queue, lane)) {
         if (idx >= (unsigned)a.nitems) break;
         const HfItem it = a.items[a.item0 + idx];
@@ -261,10 +251,13 @@ __global__ __launch_bounds__(256) void k_muscl(const HfStepArgs<typename M::R> a
             const HfBlockDev bd = a.blocks[it.block];
             if (it.block != cached) {
                 cached = it.block;
-                step_dts<R>(a, it.block, bd, true, fix ? DT_FINAL : DT_SPEC, c_step, c_pred, c_kstep, c_kpred);
                 if (fix) {
-                    double exact;
-                    c_need = needs_fix<R>(a, it.block, bd, exact);
+                    double spec;
+                    const HfProbDev pr = a.probs[bd.problem];
+                    step_dts_fix<R>(a, bd.problem, pr, bd.lb, c_step, spec, c_pred, c_kstep, c_kpred);
+                    c_need = a.skip[it.block] || spec != c_pred;
+                } else {
+                    step_dts<R>(a, it.block, bd, true, DT_SPEC, c_step, c_pred, c_kstep, c_kpred);
                 }
             }
             dt_step = c_step;

@@ -348,4 +348,49 @@ __device__ __forceinline__ void step_dts(const HfStepArgs<R>& a, int block, cons
     }
 }
 
+// step_dts of a compat MUSCL block in both views from one pass over the
+// problem's blocks: the step dt, the speculative predictor dt (DT_SPEC: every
+// block at U^n, i.e. the step's own minima) and the exact one (DT_FINAL).
+// The same operands in the same order as two step_dts calls, so the same bits.
+// The block is the lb-th of problem p (record pr): no block record is needed,
+// and every load of the pass depends only on pr, so they issue together.
+template <class R>
+__device__ __forceinline__ void step_dts_fix(const HfStepArgs<R>& a, int p, const HfProbDev& pr, int lb,
+                                             double& dt_step, double& spec, double& exact, long long& kstep,
+                                             long long& kpred)
+{
+    const int block = pr.first + lb;
+    kstep = (long long)a.calls[p] - pr.cps + 1;
+    kpred = kstep + 1 + lb;
+    double dtx = 1e16, dty = 1e16, ptx = 1e16, pty = 1e16;
+    const int end = pr.first + pr.nb;
+    for (int c = pr.first + (int)(threadIdx.x & 63); c < end; c += 64) {
+        const double cdx = a.blocks[c].dx, cdy = a.blocks[c].dy;
+        const double gx = bitsd(a.gmax[2 * c]), gy = bitsd(a.gmax[2 * c + 1]);
+        const double ox = smax(bitsd(a.imax_old[2 * c]), gx), oy = smax(bitsd(a.imax_old[2 * c + 1]), gy);
+        // final maxima of the blocks before this one (new_max with DT_FINAL),
+        // both candidates loaded at once
+        const bool fx = a.fixed[c] != 0u;
+        const unsigned long long fxx = a.imax_fix[2 * c], fxy = a.imax_fix[2 * c + 1];
+        const unsigned long long nwx = a.imax_new[2 * c], nwy = a.imax_new[2 * c + 1];
+        const double sx = cdx / ox, sy = cdy / oy;
+        dtx = smin(dtx, sx);
+        dty = smin(dty, sy);
+        if (c < block) {
+            ptx = smin(ptx, cdx / smax(bitsd(fx ? fxx : nwx), gx));
+            pty = smin(pty, cdy / smax(bitsd(fx ? fxy : nwy), gy));
+        } else {
+            ptx = smin(ptx, sx);
+            pty = smin(pty, sy);
+        }
+    }
+    dtx = wave_min(dtx);
+    dty = wave_min(dty);
+    ptx = wave_min(ptx);
+    pty = wave_min(pty);
+    dt_step = cfl_eff(a.cfl, kstep) * smin(dtx, dty);
+    spec = cfl_eff(a.cfl, kpred) * smin(dtx, dty);
+    exact = cfl_eff(a.cfl, kpred) * smin(ptx, pty);
+}
+
 }  // namespace hf
