@@ -565,7 +565,7 @@ int hf_create(const hf_case_desc* desc, hf_ctx** out)
     // items: the last round of a launch then finishes together instead of
     // waiting on a few long items.  Main launch: `rows`-row items over the
     // first 3/4 of each block's rows, then 32-row items (measured among 4..48).  A fix-up launch
-    // computes a single block on the whole GPU: 24-row items, then 12-row
+    // computes a single block on the whole GPU: 16-row items, then 8-row
     // items (measured fastest on the 8-block channel among 8..32 rows).
     auto add_rows = [&](int b, int y0, int y1, int r) {
         const HfBlockDev& B = c->blocks[b];
@@ -588,7 +588,7 @@ int hf_create(const hf_case_desc* desc, hf_ctx** out)
     if (c->scheme == HF_MUSCL_HANCOCK && c->dt_mode == HF_DT_COMPAT)
         for (int lb = 1; lb < c->maxnb; lb++) {
             const int first = (int)c->items.size();
-            const int fr = std::max(1, std::min(rows, 24)), fr2 = std::max(1, std::min(rows, 12));
+            const int fr = std::max(1, std::min(rows, 16)), fr2 = std::max(1, std::min(rows, 8));
             for (const auto& P : c->probs)
                 if (P.nb > lb) add_rows(P.first + lb, 0, split(c->blocks[P.first + lb].ny, fr), fr);
             for (const auto& P : c->probs)
