@@ -567,22 +567,33 @@ int hf_create(const hf_case_desc* desc, hf_ctx** out)
     // first 3/4 of each block's rows, then 32-row items (measured among 4..48).  A fix-up launch
     // computes a single block on the whole GPU: 16-row items, then 8-row
     // items (measured fastest on the 8-block channel among 8..32 rows).
-    auto add_rows = [&](int b, int y0, int y1, int r) {
+    // A MUSCL block whose last column strip would carry at most TR_COLS of its
+    // 60 output columns (1024 = 17 x 60 + 4) gets that band as transposed
+    // items instead: lanes along rows, the wave walking the band's columns,
+    // 60 rows per item (k_muscl's muscl_item_t; rows = -(cols + 256 * nrows)).
+    constexpr int TR_COLS = 30;
+    auto add_rows = [&](int b, int y0, int y1, int r, bool tr) {
         const HfBlockDev& B = c->blocks[b];
-        const int strips = (B.nx + W - 1) / W;
+        int strips = (B.nx + W - 1) / W;
+        const int rem = B.nx - (strips - 1) * W;
+        const bool band = tr && c->scheme == HF_MUSCL_HANCOCK && strips > 1 && rem <= TR_COLS;
+        if (band) strips--;
         for (int y = y0; y < y1; y += r)
             for (int s = 0; s < strips; s++) c->items.push_back(HfItem{b, s * W, c->g + y, std::min(r, y1 - y)});
+        if (band)
+            for (int y = y0; y < y1; y += 60)
+                c->items.push_back(HfItem{b, c->g + strips * W, c->g + y - 2, -(rem + 256 * std::min(60, y1 - y))});
     };
     auto split = [&](int ny, int r) { return std::max(0, std::min(ny, (ny * 3 / 4) / r * r)); };
     const int tail_rows = std::max(1, std::min(rows, 32));
     for (int b = 0; b < c->nb; b++) {
         c->item_begin[b] = (int)c->items.size();
-        add_rows(b, 0, c->blocks[b].ny, rows);          // contiguous per block (class-level launches)
+        add_rows(b, 0, c->blocks[b].ny, rows, false);   // contiguous per block (class-level launches)
         c->item_count[b] = (int)c->items.size() - c->item_begin[b];
     }
     const int perblock_end = (int)c->items.size();
-    for (int b = 0; b < c->nb; b++) add_rows(b, 0, split(c->blocks[b].ny, rows), rows);
-    for (int b = 0; b < c->nb; b++) add_rows(b, split(c->blocks[b].ny, rows), c->blocks[b].ny, tail_rows);
+    for (int b = 0; b < c->nb; b++) add_rows(b, 0, split(c->blocks[b].ny, rows), rows, true);
+    for (int b = 0; b < c->nb; b++) add_rows(b, split(c->blocks[b].ny, rows), c->blocks[b].ny, tail_rows, true);
     c->main_begin = perblock_end;
     c->main_items = (int)c->items.size() - perblock_end;
     if (c->scheme == HF_MUSCL_HANCOCK && c->dt_mode == HF_DT_COMPAT)
@@ -590,9 +601,10 @@ int hf_create(const hf_case_desc* desc, hf_ctx** out)
             const int first = (int)c->items.size();
             const int fr = std::max(1, std::min(rows, 16)), fr2 = std::max(1, std::min(rows, 8));
             for (const auto& P : c->probs)
-                if (P.nb > lb) add_rows(P.first + lb, 0, split(c->blocks[P.first + lb].ny, fr), fr);
+                if (P.nb > lb) add_rows(P.first + lb, 0, split(c->blocks[P.first + lb].ny, fr), fr, true);
             for (const auto& P : c->probs)
-                if (P.nb > lb) add_rows(P.first + lb, split(c->blocks[P.first + lb].ny, fr), c->blocks[P.first + lb].ny, fr2);
+                if (P.nb > lb)
+                    add_rows(P.first + lb, split(c->blocks[P.first + lb].ny, fr), c->blocks[P.first + lb].ny, fr2, true);
             c->fix_rounds.emplace_back(first, (int)c->items.size() - first);
         }
 

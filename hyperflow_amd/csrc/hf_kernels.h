@@ -203,6 +203,116 @@ __device__ __forceinline__ void muscl_item(const HfStepArgs<typename M::R>& a, c
 #undef ROW
 }
 
+// The same update for a narrow column band at a block's right edge, with the
+// roles of x and y exchanged: lane l owns padded row it.y0 + l and the wave
+// walks along the band's columns.  A block whose width leaves a last column
+// strip with only a few output columns (1024 = 17 x 60 + 4) would otherwise
+// spend a whole wave per row on them.  Every face state, flux and update is
+// the same function of the same operands as in muscl_item, in the same order,
+// so the results are identical.  Item encoding: rows = -(cols + 256 * nrows),
+// x0 = first output column (padded), y0 = padded row of lane 0; lanes 2..61
+// write, up to nrows of them.
+__device__ __forceinline__ bool item_transposed(const HfItem& it) { return it.rows < 0; }
+
+template <class M, int LIM, bool RESID>
+__device__ __forceinline__ void muscl_item_t(const HfStepArgs<typename M::R>& a, const HfItem it, const int lane,
+                                             const double dt_step, const double dt_pred,
+                                             unsigned long long* __restrict__ imax_out)
+{
+    typedef typename M::R R;
+
+    const HfBlockDev bd = a.blocks[it.block];
+    const int code = -it.rows;
+    const int cols = code & 255, nrows = code >> 8;
+    const int yraw = it.y0 + lane;
+    const bool valid = yraw < bd.ty;
+    const int yc = valid ? yraw : bd.ty - 1;
+    const bool grow = yc < 2 || yc >= bd.ty - 2;
+    const int pitch = bd.pitch, tx = bd.tx;
+    const R* Arow = a.A + bd.off + (int64_t)yc * pitch;
+    const R* Grow = a.G + bd.off + (int64_t)yc * pitch;
+    const int64_t plane = bd.plane;
+#define COL(x, u) load_cell<R>(((grow || (x) < 2 || (x) >= tx - 2) ? Grow : Arow) + (x), plane, (u))
+    const bool out = valid && lane >= 2 && lane < 62 && lane < 2 + nrows && yraw >= 2 && yraw < bd.ty - 2;
+
+    const R hx = R((0.5 * dt_pred) / bd.dx);     // half_dt_dx, muscl_hancock.cpp:248
+    const R hy = R((0.5 * dt_pred) / bd.dy);
+    const R rdx = R(1.0 / bd.dx);                  // one_dx, muscl_hancock.cpp:352
+    const R rdy = R(1.0 / bd.dy);
+    const R dtR = R(dt_step);
+    const R gam = R(a.gamma), g1 = R(a.gamma - 1.0);
+
+    const int x0 = it.x0;
+    R um[4], uc[4], up[4], un[4];
+    COL(x0 - 2, um);
+    COL(x0 - 1, uc);
+    COL(x0, up);
+    COL(x0 + 1, un);
+
+    // warm-up: R' of column x0-1, (L', R') of column x0, flux through face x0-1/2
+    R Fw[4], Rc[4];
+    {
+        R lo[4], hi[4];
+        reconstruct<M, LIM, true>(um, uc, up, hx, g1, lo, hi);    // column x0-1
+        R lo2[4];
+        reconstruct<M, LIM, true>(uc, up, un, hx, g1, lo2, Rc);   // column x0
+        const State<R> sl = make_state<M>(hi, gam, g1);
+        const State<R> sr = make_state<M>(lo2, gam, g1);
+        hllc<M, true>(sl, sr, Fw);
+    }
+    for (int k = 0; k < 4; k++) { uc[k] = up[k]; up[k] = un[k]; }
+    COL(x0 + 2, un);
+
+    R* Ob = RESID ? a.O : a.O + bd.off;
+    double mx = 0.0, my = 0.0;
+    for (int c = 0; c < cols; c++) {
+        const int x = x0 + c;
+        R nn[4];
+        COL(x + 3 < tx ? x + 3 : tx - 1, nn);     // next iteration's U(x+2)
+
+        // x direction: reconstruct column x+1, flux through face x+1/2
+        R Ln[4], Rn[4], Fe[4];
+        reconstruct<M, LIM, true>(uc, up, un, hx, g1, Ln, Rn);
+        {
+            const State<R> sl = make_state<M>(Rc, gam, g1);
+            const State<R> sr = make_state<M>(Ln, gam, g1);
+            hllc<M, true>(sl, sr, Fe);
+        }
+
+        // y direction: reconstruct cell (x, y) from lane neighbours
+        R ub[4], ut[4], Bf[4], Tf[4], Tl[4], Gs[4], Gn[4];
+        for (int k = 0; k < 4; k++) { ub[k] = shfl_up1(uc[k]); ut[k] = shfl_dn1(uc[k]); }
+        reconstruct<M, LIM, false>(ub, uc, ut, hy, g1, Bf, Tf);
+        for (int k = 0; k < 4; k++) Tl[k] = shfl_up1(Tf[k]);
+        {
+            const State<R> sl = make_state<M>(Tl, gam, g1);
+            const State<R> sr = make_state<M>(Bf, gam, g1);
+            hllc<M, false>(sl, sr, Gs);         // face y-1/2
+        }
+        for (int k = 0; k < 4; k++) Gn[k] = shfl_dn1(Gs[k]);   // face y+1/2
+
+        // calculate_spatial_update (muscl_hancock.cpp:364-371) and (dt*L)+U
+        R Lu[4], Un[4];
+        for (int k = 0; k < 4; k++) {
+            Lu[k] = rdx * (Fw[k] - Fe[k]) + rdy * (Gs[k] - Gn[k]);
+            Un[k] = dtR * Lu[k] + uc[k];
+        }
+        if (out) {
+            R* o = Ob + (int64_t)yraw * pitch + x;
+            for (int k = 0; k < 4; k++) o[k * plane] = RESID ? Lu[k] : Un[k];
+            if (!RESID) {
+                double dx_, dy_;
+                wave_speeds<M>(Un[0], Un[1], Un[2], Un[3], gam, g1, dx_, dy_);
+                mx = dmax_nan(mx, dx_);
+                my = dmax_nan(my, dy_);
+            }
+        }
+        for (int k = 0; k < 4; k++) { Fw[k] = Fe[k]; Rc[k] = Rn[k]; uc[k] = up[k]; up[k] = un[k]; un[k] = nn[k]; }
+    }
+    if (!RESID) store_maxima(imax_out, it.block, wave_max(mx), wave_max(my));
+#undef COL
+}
+
 // Persistent MUSCL-Hancock launch: waves take work items from a queue until
 // it is empty.  The main launch of a step computes every block with the
 // speculative predictor dt (every block at U^n), except blocks marked skip[b]
@@ -272,7 +382,10 @@ __global__ __launch_bounds__(256) void k_muscl(const HfStepArgs<typename M::R> a
                 if (kpred >= 1 && kpred <= a.hist_cap) a.hist[kpred - 1] = dt_pred;
             }
         }
-        muscl_item<M, LIM, RESID>(a, it, lane, dt_step, dt_pred, fix ? a.imax_fix : a.imax_new);
+        if (!RESID && item_transposed(it))
+            muscl_item_t<M, LIM, RESID>(a, it, lane, dt_step, dt_pred, fix ? a.imax_fix : a.imax_new);
+        else
+            muscl_item<M, LIM, RESID>(a, it, lane, dt_step, dt_pred, fix ? a.imax_fix : a.imax_new);
     }
 }
 

@@ -15,11 +15,18 @@ from hyperflow_amd import case as C
 pytestmark = pytest.mark.gpu
 
 
-def random_case(seed: int) -> C.Case:
+def random_case(seed: int, wide: bool = False) -> C.Case:
+    """wide: block widths of 61..90 or 121..150 cells, so that the last
+    60-column strip of a block carries 1..30 columns (a transposed band in the
+    MUSCL launches), and heights up to 140 rows (several 60-row band items)."""
     rng = np.random.default_rng(seed)
     bx, by = int(rng.integers(1, 4)), int(rng.integers(1, 4))
-    ncol = [int(rng.integers(6, 33)) for _ in range(bx)]
-    nrow = [int(rng.integers(6, 33)) for _ in range(by)]
+    if wide:
+        ncol = [int(rng.integers(1, 31)) + int(rng.choice([60, 120])) for _ in range(bx)]
+        nrow = [int(rng.integers(6, 141)) for _ in range(by)]
+    else:
+        ncol = [int(rng.integers(6, 33)) for _ in range(bx)]
+        nrow = [int(rng.integers(6, 33)) for _ in range(by)]
     order = rng.permutation(bx * by)              # list position of grid block (i, j)
     pos = {}
     for k, (j, i) in enumerate((j, i) for j in range(by) for i in range(bx)):
@@ -44,7 +51,7 @@ def random_case(seed: int) -> C.Case:
                    "left": sides["left"] if i == 0 else {"type": "Internal", "block": pos[(i - 1, j)]},
                    "right": sides["right"] if i == bx - 1 else {"type": "Internal", "block": pos[(i + 1, j)]}}
             blocks[pos[(i, j)]] = C.Block(i / bx, (i + 1) / bx, j / by, (j + 1) / by, ncol[i], nrow[j], bnd)
-    scheme = "Godunov" if rng.random() < 0.3 else "MUSCLHancock"
+    scheme = "Godunov" if rng.random() < 0.3 and not wide else "MUSCLHancock"
     limiter = None if scheme == "Godunov" else str(rng.choice(["MinBee", "VanLeer"]))
     base = C.quad_case(8, 8, seed, scheme=scheme, limiter=limiter, cfl=0.8 if scheme == "Godunov" else 0.5)
     base.blocks = blocks
@@ -58,6 +65,19 @@ def test_random_multiblock_matches_oracle(seed):
     say(f"random case {seed}: {len(case.blocks)} blocks, {case.scheme} {case.limiter}")
     _, out, hist = gpu_run(case, f, 40)
     ref, calls, _ = oracle_run(case, f, 40)
+    assert goldens.same(hist, calls), "TimeStep sequence"
+    for b in range(len(case.blocks)):
+        assert goldens.same(out[b], ref[b]), f"block {b}"
+
+
+@pytest.mark.parametrize("seed", range(100, 108))
+def test_random_wide_blocks_match_oracle(seed):
+    """Blocks whose right edge is computed by transposed band items."""
+    case = random_case(seed, wide=True)
+    f = C.perturbed_fields(case, seed)
+    say(f"random wide case {seed}: {[(b.nx, b.ny) for b in case.blocks]}, {case.limiter}")
+    _, out, hist = gpu_run(case, f, 20)
+    ref, calls, _ = oracle_run(case, f, 20)
     assert goldens.same(hist, calls), "TimeStep sequence"
     for b in range(len(case.blocks)):
         assert goldens.same(out[b], ref[b]), f"block {b}"
