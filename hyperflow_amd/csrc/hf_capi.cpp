@@ -553,6 +553,18 @@ int hf_create(const hf_case_desc* desc, hf_ctx** out)
     for (auto& P : c->probs) P.cps = c->scheme == HF_MUSCL_HANCOCK ? 1 + P.nb : 1;
     c->calls.assign(c->np, 0);
 
+    // resident waves of a persistent update launch (the work items are sized for them)
+    {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return fail(HF_EHIP, "hf_create: no usable GPU");
+        const int per_cu = c->prec == HF_FP64 ? hf::update_wgs_per_cu<double>(c->scheme, c->limiter, false)
+                                              : hf::update_wgs_per_cu<float>(c->scheme, c->limiter, false);
+        c->wg_cap = std::max(1, cus * per_cu);
+    }
+    const int64_t wave_slots = 4LL * c->wg_cap;
+
     // work items: every block in order (the main launch of a step), then,
     // for compat MUSCL problems of several blocks, the lb-th blocks of all
     // problems for each fix-up launch lb = 1, 2, ..
@@ -563,10 +575,14 @@ int hf_create(const hf_case_desc* desc, hf_ctx** out)
     c->item_count.assign(c->nb, 0);
     // Items are taken from a queue in list order, so the list ends with small
     // items: the last round of a launch then finishes together instead of
-    // waiting on a few long items.  Main launch: `rows`-row items over the
-    // first 3/4 of each block's rows, then 32-row items (measured among 4..48).  A fix-up launch
-    // computes a single block on the whole GPU: 16-row items, then 8-row
-    // items (measured fastest on the 8-block channel among 8..32 rows).
+    // waiting on a few long items (a timeline of the 8-block channel's main
+    // launch showed half of the SIMDs idle over its last fifth with a 3/4
+    // split into `rows`- and 32-row items).  Main launch: `rows`-row items,
+    // then rows/2, then 8-row items (on the channel 11/16, 1/4 and 1/16 of the
+    // rows, measured fastest among eight such plans; launches of many rounds
+    // keep the small phases to about one round's worth of work).  A
+    // fix-up launch computes a single block on the whole GPU: uniform items
+    // sized so that one round of the resident waves takes them all.
     // A MUSCL block whose last column strip would carry at most TR_COLS of its
     // 60 output columns (1024 = 17 x 60 + 4) gets that band as transposed
     // items instead: lanes along rows, the wave walking the band's columns,
@@ -584,27 +600,54 @@ int hf_create(const hf_case_desc* desc, hf_ctx** out)
             for (int y = y0; y < y1; y += 60)
                 c->items.push_back(HfItem{b, c->g + strips * W, c->g + y - 2, -(rem + 256 * std::min(60, y1 - y))});
     };
-    auto split = [&](int ny, int r) { return std::max(0, std::min(ny, (ny * 3 / 4) / r * r)); };
-    const int tail_rows = std::max(1, std::min(rows, 32));
+    // column strips of a block without its transposed band (the band is cheap)
+    auto full_strips = [&](int b) {
+        const HfBlockDev& B = c->blocks[b];
+        const int strips = (B.nx + W - 1) / W;
+        return (c->scheme == HF_MUSCL_HANCOCK && strips > 1 && B.nx - (strips - 1) * W <= TR_COLS) ? strips - 1
+                                                                                                   : strips;
+    };
     for (int b = 0; b < c->nb; b++) {
         c->item_begin[b] = (int)c->items.size();
         add_rows(b, 0, c->blocks[b].ny, rows, false);   // contiguous per block (class-level launches)
         c->item_count[b] = (int)c->items.size() - c->item_begin[b];
     }
     const int perblock_end = (int)c->items.size();
-    for (int b = 0; b < c->nb; b++) add_rows(b, 0, split(c->blocks[b].ny, rows), rows, true);
-    for (int b = 0; b < c->nb; b++) add_rows(b, split(c->blocks[b].ny, rows), c->blocks[b].ny, tail_rows, true);
+    {
+        // wave-rows of the launch; the last two phases hold about 17 and 4.25
+        // rows per resident wave (the 8-block channel: 1/4 and 1/16 of its
+        // rows), at most 1/4 and 1/16 of each block
+        double work = 0.0;
+        for (int b = 0; b < c->nb; b++) work += (double)full_strips(b) * c->blocks[b].ny;
+        const double f3 = std::min(1.0 / 16, 4.25 * (double)wave_slots / std::max(1.0, work));
+        const double f2 = std::min(0.25, 17.0 * (double)wave_slots / std::max(1.0, work));
+        const int r2 = std::max(1, std::min(rows, std::max(8, rows / 2))), r3 = std::max(1, std::min(rows, 8));
+        std::vector<int> y1(c->nb), y2(c->nb);
+        for (int b = 0; b < c->nb; b++) {
+            const int ny = c->blocks[b].ny;
+            y1[b] = std::min(ny, (int)(ny * (1.0 - f2 - f3)) / rows * rows);
+            y2[b] = std::min(ny, y1[b] + std::max(0, (int)(ny * (1.0 - f3)) - y1[b]) / r2 * r2);
+        }
+        for (int b = 0; b < c->nb; b++) add_rows(b, 0, y1[b], rows, true);
+        for (int b = 0; b < c->nb; b++) add_rows(b, y1[b], y2[b], r2, true);
+        for (int b = 0; b < c->nb; b++) add_rows(b, y2[b], c->blocks[b].ny, r3, true);
+    }
     c->main_begin = perblock_end;
     c->main_items = (int)c->items.size() - perblock_end;
     if (c->scheme == HF_MUSCL_HANCOCK && c->dt_mode == HF_DT_COMPAT)
         for (int lb = 1; lb < c->maxnb; lb++) {
+            // rows per item so that the launch's wave-rows fill about 95 % of
+            // one round of the resident waves, within 4..16 rows
+            int64_t work = 0;
+            for (const auto& P : c->probs)
+                if (P.nb > lb) {
+                    work += (int64_t)full_strips(P.first + lb) * c->blocks[P.first + lb].ny;
+                }
+            const int64_t target = std::max<int64_t>(1, wave_slots * 95 / 100);
+            const int fr = (int)std::max<int64_t>(4, std::min<int64_t>(16, (work + target - 1) / target));
             const int first = (int)c->items.size();
-            const int fr = std::max(1, std::min(rows, 16)), fr2 = std::max(1, std::min(rows, 8));
             for (const auto& P : c->probs)
-                if (P.nb > lb) add_rows(P.first + lb, 0, split(c->blocks[P.first + lb].ny, fr), fr, true);
-            for (const auto& P : c->probs)
-                if (P.nb > lb)
-                    add_rows(P.first + lb, split(c->blocks[P.first + lb].ny, fr), c->blocks[P.first + lb].ny, fr2, true);
+                if (P.nb > lb) add_rows(P.first + lb, 0, c->blocks[P.first + lb].ny, std::min(rows, fr), true);
             c->fix_rounds.emplace_back(first, (int)c->items.size() - first);
         }
 
@@ -670,14 +713,6 @@ int hf_create(const hf_case_desc* desc, hf_ctx** out)
     }
     HF_HIPC(hipMalloc(&C->d_queue, sizeof(uint32_t) * (C->maxnb + 1)));
     HF_HIPC(hipMemset(C->d_queue, 0, sizeof(uint32_t) * (C->maxnb + 1)));
-    {
-        int dev = 0, cus = 0;
-        HF_HIPC(hipGetDevice(&dev));
-        HF_HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        const int per_cu = C->prec == HF_FP64 ? hf::update_wgs_per_cu<double>(C->scheme, C->limiter, false)
-                                              : hf::update_wgs_per_cu<float>(C->scheme, C->limiter, false);
-        C->wg_cap = std::max(1, cus * per_cu);
-    }
     HF_HIPC(hipMalloc(&C->d_calls, sizeof(uint32_t) * C->np));
     HF_HIPC(hipMemset(C->d_calls, 0, sizeof(uint32_t) * C->np));
     HF_HIPC(hipHostMalloc((void**)&C->h_calls_pinned, sizeof(uint32_t) * C->np, hipHostMallocDefault));
