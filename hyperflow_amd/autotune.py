@@ -21,7 +21,7 @@ import numpy as np
 from . import native
 
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
-CANDIDATES = (8, 16, 24, 32, 48, 64, 96)
+CANDIDATES = (8, 16, 24, 32, 40, 42, 44, 48, 56, 64, 96)
 
 
 def signature(cases, precision: str, dt_mode: str) -> str:
@@ -47,13 +47,13 @@ def cached_rows(cases, precision: str = "fp64", dt_mode: str = "compat") -> int:
     return int(entry["tile_rows"]) if entry else 0
 
 
-def _time(cases, fields, precision, dt_mode, rows, steps, warmup):
+def _time(cases, fields, precision, dt_mode, rows, steps, warmup, graph=True):
     s = native.Solver(cases, precision=precision, dt_mode=dt_mode, tile_rows=rows)
     s.upload_all(fields)
-    s.run(warmup, graph=True)
+    s.run(warmup, graph=graph)
     s.synchronize()
     t = time.perf_counter()
-    s.run(steps, graph=True)
+    s.run(steps, graph=graph)
     s.synchronize()
     el = time.perf_counter() - t
     out = s.download_all()
@@ -63,17 +63,18 @@ def _time(cases, fields, precision, dt_mode, rows, steps, warmup):
 
 def tune(cases, fields: Sequence[np.ndarray], precision: str = "fp64", dt_mode: str = "compat",
          candidates: Sequence[int] = CANDIDATES, steps: int = 20, warmup: int = 3,
-         save: bool = True, verbose: bool = False) -> dict:
-    """Times every candidate on `steps` graph-captured steps from the given
-    fields and returns {"tile_rows", "ms_per_step", "timings", "rejected"}."""
+         save: bool = True, verbose: bool = False, graph: bool = True) -> dict:
+    """Times every candidate on `steps` steps from the given fields (HIP-graph
+    captured, or eager launches as the workload's benchmark runs them) and
+    returns {"tile_rows", "ms_per_step", "timings", "rejected"}."""
     if not isinstance(cases, (list, tuple)):
         cases = [cases]
     ref_rows = candidates[0]
-    ref_time, ref_out = _time(cases, fields, precision, dt_mode, ref_rows, steps, warmup)
+    ref_time, ref_out = _time(cases, fields, precision, dt_mode, ref_rows, steps, warmup, graph)
     timings = {ref_rows: ref_time / steps * 1e3}
     rejected: List[int] = []
     for rows in candidates[1:]:
-        el, out = _time(cases, fields, precision, dt_mode, rows, steps, warmup)
+        el, out = _time(cases, fields, precision, dt_mode, rows, steps, warmup, graph)
         same = all(np.array_equal(a, b, equal_nan=True) for a, b in zip(out, ref_out))
         if not same:
             rejected.append(rows)
@@ -93,29 +94,30 @@ def tune(cases, fields: Sequence[np.ndarray], precision: str = "fp64", dt_mode: 
 
 
 def _workloads(names):
-    """The benchmark workloads by name: (label, cases, fields, precision, dt_mode)."""
+    """The benchmark workloads by name: (label, cases, fields, precision,
+    dt_mode, graph), graph as bench.py launches the workload."""
     from . import case as C
     out = []
     for name in names:
         if name == "c1":
             c = C.config1(0)
-            out.append(("c1", [c], [c.initial_fields()[0]], "fp64", "compat"))
+            out.append(("c1", [c], [c.initial_fields()[0]], "fp64", "compat", False))
         elif name == "c2":
             c = C.config2(1024)
-            out.append(("c2", [c], c.initial_fields(), "fp64", "compat"))
+            out.append(("c2", [c], c.initial_fields(), "fp64", "compat", False))
         elif name in ("c3", "c3_fp32"):
             c = C.config3(0, 8192)
             f = C.perturbed_fields(c, 0)
             for prec in (("fp64", "fp32") if name == "c3" else ("fp32",)):
-                out.append((f"c3_{prec}", [c], f, prec, "compat"))
+                out.append((f"c3_{prec}", [c], f, prec, "compat", False))
         elif name == "c4":
             cases = [C.quad_case(nx, ny, 1000 + k, scheme="MUSCLHancock", limiter="VanLeer", cfl=0.5)
                      for k, (nx, ny) in enumerate(C.batched_sizes(512, 0))]
-            out.append(("c4", cases, [c.initial_fields()[0] for c in cases], "fp32", "compat"))
+            out.append(("c4", cases, [c.initial_fields()[0] for c in cases], "fp32", "compat", True))
         elif name == "c5":
             for n in (1024, 1280, 1536, 1792, 2048):
                 c = C.config3(0, n)
-                out.append((f"c5_{n}", [c], C.perturbed_fields(c, 0), "fp32", "uniform"))
+                out.append((f"c5_{n}", [c], C.perturbed_fields(c, 0), "fp32", "uniform", True))
         else:
             raise SystemExit(f"unknown workload {name}")
     return out
@@ -126,9 +128,10 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser(prog="python -m hyperflow_amd.autotune",
                                  description="tune tile rows of the benchmark workloads into the cache")
     ap.add_argument("workloads", nargs="*", default=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--steps", type=int, default=20, help="timed steps per candidate")
     args = ap.parse_args(argv)
-    for label, cases, fields, prec, mode in _workloads(args.workloads):
-        r = tune(cases, fields, precision=prec, dt_mode=mode)
+    for label, cases, fields, prec, mode, graph in _workloads(args.workloads):
+        r = tune(cases, fields, precision=prec, dt_mode=mode, steps=args.steps, graph=graph)
         print(f"{label}: tile_rows {r['tile_rows']} ({r['ms_per_step']:.4f} ms/step) {r['timings']}", flush=True)
     return 0
 
