@@ -98,7 +98,7 @@ def learned_line(seeds=(5000, 5001, 5002), coarse_steps: int = 200):
     the 2x2-averaged 2048^2 solution, and wall times (HIP graphs, state in
     HBM), for plain runs at 1024..1792^2 and the corrected 1024^2 run."""
     from hyperflow_amd import learned as L
-    L.ensure_tuned((1024, 1280, 1536, 1792, 2048), log=lambda m: print(m, file=sys.stderr))
+    L.ensure_tuned((1024, 1152, 1280, 1536, 1792, 2048), log=lambda m: print(m, file=sys.stderr))
     net = L.load()
     meta = L.load_meta()
     every, start = int(meta["every"]), int(meta["start"])
@@ -114,6 +114,8 @@ def learned_line(seeds=(5000, 5001, 5002), coarse_steps: int = 200):
             "learned_l1": lr["l1"], "learned_seconds": lr["seconds"], "plain_resolutions": summ["resolutions"],
             "plain_l1": summ["plain_l1"], "plain_seconds": summ["plain_seconds"],
             "plain_2048_seconds": summ["fine_seconds"], "plain_n_at_equal_l1": lr["plain_n_at_equal_l1"],
+            "plain_seconds_at_equal_l1": lr["plain_seconds_at_equal_l1"],
+            "speedup_at_equal_l1_n3_fit": lr["speedup_at_equal_l1_fit"],
             "l1_gain_vs_plain_1024": lr["l1_gain_vs_plain_1024"], "speedup_vs_plain_2048": lr["speedup_vs_plain_2048"],
             "vs_baseline": None}
 

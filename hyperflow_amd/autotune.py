@@ -115,7 +115,7 @@ def _workloads(names):
                      for k, (nx, ny) in enumerate(C.batched_sizes(512, 0))]
             out.append(("c4", cases, [c.initial_fields()[0] for c in cases], "fp32", "compat", True))
         elif name == "c5":
-            for n in (1024, 1280, 1536, 1792, 2048):
+            for n in (1024, 1152, 1280, 1536, 1792, 2048):
                 c = C.config3(0, n)
                 out.append((f"c5_{n}", [c], C.perturbed_fields(c, 0), "fp32", "uniform", True))
         else:

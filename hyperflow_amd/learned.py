@@ -367,16 +367,18 @@ def truth(icf: np.ndarray, coarse_steps: int, seed: int = 0):
     return rho, secs[-1]
 
 
-def evaluate(net, seeds, coarse_steps: int = 200, resolutions=(1024, 1280, 1536, 1792), every=(8,), start: int = 0,
-             log=print):
+def evaluate(net, seeds, coarse_steps: int = 200, resolutions=(1024, 1152, 1280, 1536, 1792), every=(8,),
+             start: int = 0, log=print):
     """Density L1 against the restricted fine solution, and wall times.
 
     Every run starts from the conservative remap of the same 2048^2 initial
-    state.  The plain resolution n* that matches the learned L1 is
-    interpolated in log L1 - log n between the measured plain runs (time
-    scales as n^3 at a fixed Courant number: n^2 cells times n steps); the
-    learned speed-up at equal L1 is time(n*) / time(learned).  The learned run
-    corrects every `every` coarse steps from step `start` on."""
+    state.  The plain resolution n* that matches the learned L1, and its wall
+    time, are interpolated in log-log between the two measured plain runs that
+    bracket the learned L1 (outside the measured range: from least-squares
+    fits, L1 ~ n^b and time ~ n^3, n^2 cells times n steps at a fixed Courant
+    number); the learned speed-up at equal L1 is time(n*) / time(learned).
+    The learned run corrects every `every` coarse steps from step `start` on.
+    Resolutions must be multiples of 128 (whole numbers of steps)."""
     rows = []
     for seed in seeds:
         icf = fine_ic(seed)
@@ -394,6 +396,21 @@ def evaluate(net, seeds, coarse_steps: int = 200, resolutions=(1024, 1280, 1536,
     return rows, summary
 
 
+def equal_l1_point(ns, l1s, ts, l1):
+    """(n*, time(n*)) of the plain runs at density L1 `l1`, interpolated
+    linearly in log-log between the two measured resolutions whose L1 brackets
+    it; (nan, nan) when no neighbouring pair brackets it."""
+    ns, l1s, ts = (np.asarray(v, np.float64) for v in (ns, l1s, ts))
+    for i in range(len(ns) - 1):
+        hi, lo = l1s[i], l1s[i + 1]
+        if hi >= l1 >= lo and hi > lo:
+            w = np.log(hi / l1) / np.log(hi / lo)
+            n = np.exp(np.log(ns[i]) + w * (np.log(ns[i + 1]) - np.log(ns[i])))
+            t = np.exp(np.log(ts[i]) + w * (np.log(ts[i + 1]) - np.log(ts[i])))
+            return float(n), float(t)
+    return float("nan"), float("nan")
+
+
 def summarize(rows, resolutions, every):
     ns = np.array(resolutions, np.float64)
     l1p = np.array([[r["plain"][n]["l1"] for n in resolutions] for r in rows]).mean(axis=0)
@@ -406,10 +423,14 @@ def summarize(rows, resolutions, every):
     for e in every:
         l1 = float(np.mean([r["learned"][e]["l1"] for r in rows]))
         t = float(np.mean([r["learned"][e]["seconds"] for r in rows]))
-        n_eq = float(np.exp((np.log(l1) - a) / b)) if b < 0 else float("nan")
-        t_eq = float(np.exp(c + 3.0 * np.log(n_eq))) if np.isfinite(n_eq) else float("nan")
+        n_fit = float(np.exp((np.log(l1) - a) / b)) if b < 0 else float("nan")
+        t_fit = float(np.exp(c + 3.0 * np.log(n_fit))) if np.isfinite(n_fit) else float("nan")
+        n_eq, t_eq = equal_l1_point(ns, l1p, tp, l1)
+        if not np.isfinite(n_eq):
+            n_eq, t_eq = n_fit, t_fit
         out["learned"][e] = {"l1": l1, "seconds": t, "plain_n_at_equal_l1": n_eq,
                              "plain_seconds_at_equal_l1": t_eq, "speedup_at_equal_l1": t_eq / t,
+                             "plain_n_at_equal_l1_fit": n_fit, "speedup_at_equal_l1_fit": t_fit / t,
                              "l1_gain_vs_plain_1024": float(l1p[0] / l1), "speedup_vs_plain_2048": out["fine_seconds"] / t}
     out["l1_slope"] = float(b)
     return out
@@ -472,7 +493,7 @@ def main(argv=None):
     e.add_argument("--start", type=int, default=None, help="first corrected step; default: stored with the weights")
     e.add_argument("--shock-threshold", type=float, default=None)
     e.add_argument("--max-output", type=float, default=None, help="fallback bound on |network output|")
-    e.add_argument("--resolutions", type=int, nargs="+", default=[1024, 1280, 1536, 1792])
+    e.add_argument("--resolutions", type=int, nargs="+", default=[1024, 1152, 1280, 1536, 1792])
     e.add_argument("--json", default=None)
     args = ap.parse_args(argv)
 
