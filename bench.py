@@ -9,19 +9,25 @@ and the eight fused per-block MUSCL-HLLC updates.  The timed region starts
 with the state resident in HBM.  Prints one JSON line.
 
 Other workloads of the same path (not the headline line):
-    --workload c1   Godunov + HLLC, 400x400 FP64, transmissive, CFL 0.8
-    --workload c3   MUSCL-Hancock + Van Leer, 8192x8192 (--precision fp32|fp64)
-    --workload c4   512 independent problems of U[64,512]^2 cells, MUSCL FP32,
-                    one persistent launch per step, HIP graph; also times
-                    looping the same problems one context at a time
-    --workload c5   learned coarse correction: 1024^2 MUSCL FP32 + the committed
-                    CNN against plain runs at 1024..1792^2 and the 2048^2
-                    solution (density L1, wall time, speed-up at equal L1)
-    --workload copy a 1 GiB device copy: measured copy bandwidth, and the
-                    calibration run for the HBM traffic counters
-    --workload u1   unstructured Godunov (fvm_2d_uns), FP64: the shipped
-                    scramjet case on its coarse block mesh refined --refine
-                    times per direction (default 8: 1 469 440 cells)
+    --workload godunov-quadrants
+        Godunov + HLLC, 400x400 FP64, transmissive, CFL 0.8
+    --workload muscl-quadrants
+        MUSCL-Hancock + Van Leer, 8192x8192 (--precision fp32|fp64)
+    --workload batch
+        512 independent problems of U[64,512]^2 cells, MUSCL FP32, one
+        persistent launch per step, HIP graph; also times looping the same
+        problems one context at a time
+    --workload learned
+        learned coarse correction: 1024^2 MUSCL FP32 + the committed CNN
+        against plain runs at 1024..1792^2 and the 2048^2 solution (density
+        L1, wall time, speed-up at equal L1)
+    --workload copy
+        a 1 GiB device copy: measured copy bandwidth, and the calibration run
+        for the HBM traffic counters
+    --workload unstructured
+        unstructured Godunov (fvm_2d_uns), FP64: the shipped scramjet case on
+        its coarse block mesh refined --refine times per direction (default 8:
+        1 469 440 cells)
 """
 from __future__ import annotations
 
@@ -51,7 +57,7 @@ def cpu_baseline(sample_n: int = 256, sample_steps: int = 5):
     import hfo
     from hyperflow_amd import case as C
 
-    case = C.config2(sample_n)
+    case = C.muscl_step_channel(sample_n)
     cells = case.interior_cells * sample_steps
     sample = f"step channel 8 blocks x {sample_n}^2, MUSCL MinBee FP64, {sample_steps} steps"
     if hfo.have_reference():
@@ -197,21 +203,21 @@ def uns_line(steps: int, warmup: int, refine: int, with_cpu: bool):
 
 def build_workload(name: str, n: int, precision: str, seed: int = 0):
     from hyperflow_amd import case as C
-    if name == "c1":
-        case = C.config1(seed)
+    if name == "godunov-quadrants":
+        case = C.godunov_quadrants(seed)
         return [case], [case.initial_fields()[0]], "Godunov+HLLC FP64, 400x400, transmissive, CFL 0.8"
-    if name == "c2":
-        case = C.config2(n)
+    if name == "step-channel":
+        case = C.muscl_step_channel(n)
         return [case], case.initial_fields(), (
             f"MUSCL-Hancock+MinBee+HLLC {precision.upper()}, 8 blocks of {n}x{n} step channel (Dirichlet inflow, "
             "reflective walls, transmissive outflow), CFL 0.5, reference-compatible dt")
-    if name == "c3":
+    if name == "muscl-quadrants":
         n3 = 8192 if n == 1024 else n
-        case = C.config3(seed, n3)
+        case = C.muscl_quadrants(seed, n3)
         return [case], C.perturbed_fields(case, seed), (
             f"MUSCL-Hancock+VanLeer+HLLC {precision.upper()}, single block {n3}x{n3}, seeded quadrants with "
             "sinusoidal density perturbation, CFL 0.5")
-    if name == "c4":
+    if name == "batch":
         cases = [C.quad_case(nx, ny, 1000 + k, scheme="MUSCLHancock", limiter="VanLeer", cfl=0.5)
                  for k, (nx, ny) in enumerate(C.batched_sizes(512, seed))]
         return cases, [c.initial_fields()[0] for c in cases], (
@@ -248,9 +254,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "c5", "copy", "u1"], default="c2")
-    ap.add_argument("--refine", type=int, default=8, help="u1: cells per coarse cell and direction")
-    ap.add_argument("--n", type=int, default=1024, help="cells per block side (c2), grid side (c3)")
+    ap.add_argument("--workload", choices=["godunov-quadrants", "step-channel", "muscl-quadrants", "batch", "learned", "copy", "unstructured"], default="step-channel")
+    ap.add_argument("--refine", type=int, default=8, help="unstructured: cells per coarse cell and direction")
+    ap.add_argument("--n", type=int, default=1024, help="cells per block side (step-channel), grid side (muscl-quadrants)")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default=None)
     ap.add_argument("--tile-rows", type=int, default=None, help="default: autotuned value if cached")
     ap.add_argument("--graph", action="store_true", help="capture the timed steps in a HIP graph")
@@ -265,19 +271,19 @@ def main():
     if args.workload == "copy":
         print(json.dumps(copy_bandwidth()), flush=True)
         return
-    if args.workload == "c5":
+    if args.workload == "learned":
         print(json.dumps(learned_line()), flush=True)
         return
-    if args.workload == "u1":
+    if args.workload == "unstructured":
         print(json.dumps(uns_line(args.steps, args.warmup, args.refine, not args.no_cpu_baseline)), flush=True)
         return
 
     from hyperflow_amd import autotune, native
 
-    precision = args.precision or ("fp32" if args.workload == "c4" else "fp64")
+    precision = args.precision or ("fp32" if args.workload == "batch" else "fp64")
     cases, fields, workload_text = build_workload(args.workload, args.n, precision)
     tile_rows = args.tile_rows if args.tile_rows is not None else autotune.cached_rows(cases, precision)
-    graph = args.graph or args.workload == "c4"
+    graph = args.graph or args.workload == "batch"
 
     solver = native.Solver(cases, precision=precision, tile_rows=tile_rows)
     solver.upload_all(fields)
@@ -306,7 +312,7 @@ def main():
         cells_per_launch = cells
     alg_bytes = cells_per_launch * bpc
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
-    # counters were taken at the default sizes (1024 per block side, 8192 for c3)
+    # counters were taken at the default sizes (1024 per block side, 8192 for muscl-quadrants)
     traffic = traffic_from_profiles(f"{args.workload}_{precision}") if args.n == 1024 else None
     kernel = ("k_muscl (all update launches of a step: main + fix-ups; bytes = the step's cells / launches)"
               if cases[0].scheme == "MUSCLHancock" else "k_godunov")
@@ -324,7 +330,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if precision == "fp64" else "f32",
         "data": ("synthetic (the channel filled with its Mach 3 inflow state, as in the reference case)"
-                 if args.workload == "c2" else "synthetic (seeded quadrant states of the named workload)"),
+                 if args.workload == "step-channel" else "synthetic (seeded quadrant states of the named workload)"),
         "config": {"workload": workload_text, "blocks": sum(len(c.blocks) for c in cases),
                    "interior_cells": cells, "scheme": cases[0].scheme, "limiter": cases[0].limiter,
                    "riemann": "HLLC", "cfl": cases[0].cfl, "tile_rows": tile_rows or "default",
@@ -336,12 +342,12 @@ def main():
                      "launches": launches, "algorithmic_bytes_per_launch": int(alg_bytes),
                      "bytes_per_cell": bpc},
     }
-    if args.workload == "c4":
+    if args.workload == "batch":
         solver.close()
         loop = looped_baseline(cases, fields, precision, args.steps)
         line["looped_per_problem_s"] = round(loop, 4)
         line["batched_speedup_vs_looped"] = round(loop / wall, 2)
-    if not args.no_cpu_baseline and args.workload == "c2":
+    if not args.no_cpu_baseline and args.workload == "step-channel":
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)
 

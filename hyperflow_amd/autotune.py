@@ -99,25 +99,25 @@ def _workloads(names):
     from . import case as C
     out = []
     for name in names:
-        if name == "c1":
-            c = C.config1(0)
-            out.append(("c1", [c], [c.initial_fields()[0]], "fp64", "compat", False))
-        elif name == "c2":
-            c = C.config2(1024)
-            out.append(("c2", [c], c.initial_fields(), "fp64", "compat", False))
-        elif name in ("c3", "c3_fp32"):
-            c = C.config3(0, 8192)
+        if name == "godunov-quadrants":
+            c = C.godunov_quadrants(0)
+            out.append(("godunov-quadrants", [c], [c.initial_fields()[0]], "fp64", "compat", False))
+        elif name == "step-channel":
+            c = C.muscl_step_channel(1024)
+            out.append(("step-channel", [c], c.initial_fields(), "fp64", "compat", False))
+        elif name in ("muscl-quadrants", "muscl-quadrants-fp32"):
+            c = C.muscl_quadrants(0, 8192)
             f = C.perturbed_fields(c, 0)
-            for prec in (("fp64", "fp32") if name == "c3" else ("fp32",)):
-                out.append((f"c3_{prec}", [c], f, prec, "compat", False))
-        elif name == "c4":
+            for prec in (("fp64", "fp32") if name == "muscl-quadrants" else ("fp32",)):
+                out.append((f"muscl-quadrants-{prec}", [c], f, prec, "compat", False))
+        elif name == "batch":
             cases = [C.quad_case(nx, ny, 1000 + k, scheme="MUSCLHancock", limiter="VanLeer", cfl=0.5)
                      for k, (nx, ny) in enumerate(C.batched_sizes(512, 0))]
-            out.append(("c4", cases, [c.initial_fields()[0] for c in cases], "fp32", "compat", True))
-        elif name == "c5":
+            out.append(("batch", cases, [c.initial_fields()[0] for c in cases], "fp32", "compat", True))
+        elif name == "learned":
             for n in (1024, 1152, 1280, 1536, 1792, 2048):
-                c = C.config3(0, n)
-                out.append((f"c5_{n}", [c], C.perturbed_fields(c, 0), "fp32", "uniform", True))
+                c = C.muscl_quadrants(0, n)
+                out.append((f"learned-{n}", [c], C.perturbed_fields(c, 0), "fp32", "uniform", True))
         else:
             raise SystemExit(f"unknown workload {name}")
     return out
@@ -127,7 +127,7 @@ def main(argv=None) -> int:
     import argparse
     ap = argparse.ArgumentParser(prog="python -m hyperflow_amd.autotune",
                                  description="tune tile rows of the benchmark workloads into the cache")
-    ap.add_argument("workloads", nargs="*", default=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("workloads", nargs="*", default=["godunov-quadrants", "step-channel", "muscl-quadrants", "batch", "learned"])
     ap.add_argument("--steps", type=int, default=20, help="timed steps per candidate")
     args = ap.parse_args(argv)
     for label, cases, fields, prec, mode, graph in _workloads(args.workloads):

@@ -256,7 +256,7 @@ def quad_case(nx: int, ny: int, seed: int, scheme: str = "Godunov", limiter: Opt
     return Case(blocks=[blk], initial_condition=ic, scheme=scheme, limiter=limiter, gamma=gamma, cfl=cfl)
 
 
-def config1(seed: int = 0, n: int = 400) -> Case:
+def godunov_quadrants(seed: int = 0, n: int = 400) -> Case:
     """Godunov + HLLC, one n x n block on [0,1]^2, transmissive, CFL 0.8."""
     return quad_case(n, n, seed, scheme="Godunov", cfl=0.8)
 
@@ -292,7 +292,7 @@ def step_channel(n: int = 1024, limiter: str = "MinBee", cfl: float = 0.5) -> Ca
                 domain={"x_left": 0.0, "x_right": 5.0, "y_bottom": 0.0, "y_top": 2.0})
 
 
-def config2(n: int = 1024) -> Case:
+def muscl_step_channel(n: int = 1024) -> Case:
     """MUSCL-Hancock + MinBee + HLLC, 8 blocks of n x n, step channel, CFL 0.5."""
     return step_channel(n, "MinBee", 0.5)
 
@@ -353,7 +353,7 @@ def nonphysical_fields(case: Case, seed: int, block: int = 0) -> List[np.ndarray
     return fields
 
 
-def config3(seed: int = 0, n: int = 8192, limiter: str = "VanLeer") -> Case:
+def muscl_quadrants(seed: int = 0, n: int = 8192, limiter: str = "VanLeer") -> Case:
     """MUSCL-Hancock + Van Leer, one n x n block, transmissive, CFL 0.5
     (fields come from perturbed_fields)."""
     c = quad_case(n, n, seed, scheme="MUSCLHancock", limiter=limiter, cfl=0.5)

@@ -22,7 +22,7 @@ Training runs on the GPU with the library's own kernels (hf_net_train_step:
 BF16 MFMA forward and backward, Adam).  torch only provides device memory.
 
     python -m hyperflow_amd.learned train [--seeds 40] [--every 4] [--rounds 2] [--steps 30000]
-    python -m hyperflow_amd.learned evaluate [--weights hyperflow_amd/learned_c5.npz]
+    python -m hyperflow_amd.learned evaluate [--weights hyperflow_amd/learned_weights.npz]
 """
 from __future__ import annotations
 
@@ -45,7 +45,7 @@ from . import case as C
 from . import native
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-WEIGHTS = os.path.join(HERE, "learned_c5.npz")
+WEIGHTS = os.path.join(HERE, "learned_weights.npz")
 COURANT = 0.4 / 6.6
 COARSE, FINE = 1024, 2048
 GAMMA = 1.4
@@ -58,7 +58,7 @@ def dt_for(n: int) -> float:
 
 
 def make_case(n: int, seed: int) -> C.Case:
-    return C.config3(seed, n)
+    return C.muscl_quadrants(seed, n)
 
 
 def solver(n: int, seed: int, fields=None) -> native.Solver:

@@ -36,7 +36,7 @@ def test_step_channel_internal_faces_are_consistent():
 
 
 def test_perturbed_fields_keep_density_positive():
-    c = C.config3(2, 32)
+    c = C.muscl_quadrants(2, 32)
     f = C.perturbed_fields(c, 2)[0]
     g = c.ghost
     assert (f[0, g:-g, g:-g] > 0).all()

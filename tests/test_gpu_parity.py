@@ -1,8 +1,8 @@
 """Parity of the gfx950 FP64 path with the reference (golden vectors) and the
 CPU oracle.  Integer-exact expectations: the FP64 kernels evaluate the
 reference's formulas in its order, so results must be identical (sign of zero
-aside); the north-star bar (rel L-inf <= 1e-12 after 1 step, <= 1e-9 after
-100) is asserted on top."""
+aside); the FP64 accuracy bar (rel L-inf <= 1e-12 after 1 step, <= 1e-9
+after 100) is asserted on top."""
 import numpy as np
 import pytest
 
@@ -91,7 +91,7 @@ def test_class_level_api_matches_oracle(scheme, lim):
 def test_config1_full_size_against_oracle():
     """Godunov + HLLC 400x400, transmissive, CFL 0.8, 100 steps."""
     for seed in (0, 1):
-        case = C.config1(seed)
+        case = C.godunov_quadrants(seed)
         f = case.initial_fields()
         for n in (1, 100):
             say(f"config1 seed {seed}: {n} steps")
@@ -106,7 +106,7 @@ def test_config1_full_size_against_oracle():
 def test_step_channel_reduced_against_oracle():
     """The config-2 geometry (8 blocks, Dirichlet/Reflective/Internal/
     Transmissive, MUSCL MinBee, compat dt) at 64x64 per block, 500 steps."""
-    case = C.config2(64)
+    case = C.muscl_step_channel(64)
     f = case.initial_fields()
     say("step channel 64: 500 steps")
     _, out, hist = gpu_run(case, f, 500)
@@ -117,7 +117,7 @@ def test_step_channel_reduced_against_oracle():
 
 
 def test_config2_full_size_first_steps_against_oracle():
-    case = C.config2(1024)
+    case = C.muscl_step_channel(1024)
     f = case.initial_fields()
     say("config2 full size: 3 steps")
     _, out, hist = gpu_run(case, f, 3)
