@@ -179,7 +179,7 @@ def uns_line(steps: int, warmup: int, refine: int, with_cpu: bool):
                                f"direction ({n} quadrilateral cells), Godunov + rotated HLLC, forward Euler",
                    "cells": n, "cfl": cfl},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(f"u1_r{refine}"),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(f"unstructured_r{refine}"),
                      "kernel": "k_uns_step", "avg_launch_ms": round(avg_ms, 5), "launches": launches,
                      "algorithmic_bytes_per_launch": n * bpc, "bytes_per_cell": bpc},
     }
@@ -264,8 +264,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported by this build "
-              "(the path would shard by blocks with a halo exchange and a dt min-allreduce, not measured here)",
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the path would shard by blocks with a halo exchange and a dt min-allreduce; not implemented)",
               file=sys.stderr)
         sys.exit(2)
     if args.workload == "copy":

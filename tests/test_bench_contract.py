@@ -1,0 +1,49 @@
+"""bench.py's command line and the committed measurements it reads (no GPU):
+every workload name resolves, the update-kernel workloads find their HBM
+traffic figures at the default sizes, and more than one GPU is refused."""
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+import bench  # noqa: E402
+
+# workload name and precision as bench.py's default run of each uses them
+TRAFFIC_KEYS = [("step-channel", "fp64"), ("muscl-quadrants", "fp64"), ("muscl-quadrants", "fp32"),
+                ("godunov-quadrants", "fp64")]
+
+
+@pytest.mark.parametrize("workload,precision", TRAFFIC_KEYS)
+def test_traffic_figures_exist_for_the_update_workloads(workload, precision):
+    t = bench.traffic_from_profiles(f"{workload}_{precision}")
+    assert isinstance(t, int) and t > 0
+
+
+def test_traffic_figure_exists_for_the_unstructured_workload():
+    assert bench.traffic_from_profiles("unstructured_r8") > 0
+
+
+def test_traffic_keys_are_workload_names():
+    with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as f:
+        keys = [k for k in json.load(f) if not k.startswith("_")]
+    names = {"step-channel", "muscl-quadrants", "godunov-quadrants", "batch", "learned", "unstructured"}
+    for k in keys:
+        assert k.rsplit("_", 1)[0] in names, k
+
+
+@pytest.mark.parametrize("name", ["godunov-quadrants", "step-channel", "muscl-quadrants", "batch"])
+def test_workloads_build(name):
+    cases, fields, text = bench.build_workload(name, 16, "fp64")
+    assert len(cases) == len(fields) or name == "step-channel"
+    assert text
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in (r.stdout + r.stderr) or "1 GPU" in (r.stdout + r.stderr)
