@@ -53,6 +53,24 @@ def test_tile_rows_do_not_change_results(rows):
     assert goldens.same(out[0], ref[0])
 
 
+@pytest.mark.parametrize("rows", [1, 2, 5])
+@pytest.mark.parametrize("kind", ["godunov", "channel"])
+def test_small_tile_rows_match_oracle(kind, rows):
+    """Items of a few rows: Godunov on one block, and the 8-block channel whose
+    compat dt needs fix-up launches (their items are capped at the tile rows)."""
+    if kind == "godunov":
+        case = C.quad_case(97, 61, 12, scheme="Godunov", cfl=0.8)
+    else:
+        case = C.step_channel(18, "MinBee", 0.5)
+    f = case.initial_fields()
+    say(f"small tile rows: {kind} rows {rows}")
+    _, out, hist = gpu_run(case, f, 15, tile_rows=rows)
+    ref, calls, _ = oracle_run(case, f, 15)
+    for b in range(len(ref)):
+        assert goldens.same(out[b], ref[b]), f"block {b}"
+    assert goldens.same(hist, calls)
+
+
 @pytest.mark.parametrize("scheme,lim", [("Godunov", None), ("MUSCLHancock", "MinBee"), ("MUSCLHancock", "VanLeer")])
 def test_class_level_api_matches_oracle(scheme, lim):
     """Mesh::apply_bcs, TimeStep, Scheme::operator() and the per-block
