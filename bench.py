@@ -47,6 +47,28 @@ BYTES_PER_CELL = {"fp64": 64, "fp32": 32}   # read U^n + write U^{n+1}, 4 conser
 # the update launches of every 10th timed step carry HIP timing events (stamped by the launch itself):
 # the average launch duration is sampled over the whole timed region at a tenth of the timing cost
 PROFILE_EVERY = 10
+# The FP64 MUSCL-Hancock row loop of k_muscl (one row of 60 output cells per
+# wave), counted in the gfx950 assembly: MinBee 1 397 VALU instructions, Van
+# Leer 1 429, 60 of them quarter-rate (v_rcp_f64 / v_rsq_f64).  A wave64 VALU
+# instruction holds its SIMD for 4 cycles (16 for quarter rate); 1 024 SIMDs
+# at 2.4 GHz.
+VALU_ROW = {("MUSCLHancock", "MinBee", "fp64"): (1397, 60), ("MUSCLHancock", "VanLeer", "fp64"): (1429, 60)}
+VALU_SIMDS, VALU_GHZ, VALU_CELLS_PER_ROW = 1024, 2.4, 60
+
+
+def valu_bound(scheme, limiter, precision, cells_per_s):
+    """The update kernel's cell rate against its VALU issue bound (None where
+    the row loop has not been counted)."""
+    key = (scheme, limiter, precision)
+    if key not in VALU_ROW:
+        return None
+    n, quarter = VALU_ROW[key]
+    cycles = (n - quarter) * 4 + quarter * 16
+    bound = VALU_SIMDS * VALU_GHZ * 1e9 / cycles * VALU_CELLS_PER_ROW / 1e6
+    achieved = cells_per_s / 1e6
+    return {"unit": "Mcell-updates/s", "achieved": round(achieved, 1), "bound": round(bound, 1),
+            "frac": round(achieved / bound, 4), "valu_per_row": n, "quarter_rate_per_row": quarter,
+            "cells_per_row": VALU_CELLS_PER_ROW}
 
 
 def cpu_baseline(sample_n: int = 256, sample_steps: int = 5):
@@ -342,6 +364,10 @@ def main():
                      "launches": launches, "algorithmic_bytes_per_launch": int(alg_bytes),
                      "bytes_per_cell": bpc},
     }
+    if launches:
+        valu = valu_bound(cases[0].scheme, cases[0].limiter, precision, cells_per_launch / (avg_ms * 1e-3))
+        if valu:
+            line["roofline"]["valu"] = valu
     if args.workload == "batch":
         solver.close()
         loop = looped_baseline(cases, fields, precision, args.steps)

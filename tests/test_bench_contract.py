@@ -47,3 +47,12 @@ def test_more_than_one_gpu_is_refused():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode != 0
     assert "one GPU" in (r.stdout + r.stderr) or "1 GPU" in (r.stdout + r.stderr)
+
+
+def test_valu_bound_of_the_fp64_muscl_loop():
+    """About 23.4 Gcell-updates/s: 1 337 full-rate and 60 quarter-rate
+    instructions per 60-cell wave-row on 1 024 SIMDs at 2.4 GHz."""
+    v = bench.valu_bound("MUSCLHancock", "MinBee", "fp64", 18.0e9)
+    assert 23300 < v["bound"] < 23450
+    assert abs(v["frac"] - 18000.0 / v["bound"]) < 1e-4
+    assert bench.valu_bound("Godunov", None, "fp64", 1e9) is None
