@@ -320,6 +320,7 @@ def main():
     t1 = time.perf_counter()
     launches, kernel_ms = solver.kernel_stats()
     solver.profile(False)
+    solver.close()          # release the context before anything else runs (no teardown at interpreter exit)
 
     wall = t1 - t0
     cells = sum(c.interior_cells for c in cases)
@@ -336,7 +337,7 @@ def main():
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
     # counters were taken at the default sizes (1024 per block side, 8192 for muscl-quadrants)
     traffic = traffic_from_profiles(f"{args.workload}_{precision}") if args.n == 1024 else None
-    kernel = ("k_muscl (all update launches of a step: main + fix-ups; bytes = the step's cells / launches)"
+    kernel = ("k_muscl (the step's update launch: main list and fix-up rounds)"
               if cases[0].scheme == "MUSCLHancock" else "k_godunov")
 
     line = {
@@ -369,7 +370,6 @@ def main():
         if valu:
             line["roofline"]["valu"] = valu
     if args.workload == "batch":
-        solver.close()
         loop = looped_baseline(cases, fields, precision, args.steps)
         line["looped_per_problem_s"] = round(loop, 4)
         line["batched_speedup_vs_looped"] = round(loop / wall, 2)

@@ -1,0 +1,121 @@
+"""Error of the FP32 bulk path against the FP64 path, per benchmark
+configuration.
+
+The FP64 path is the reference's arithmetic bit for bit (every field and every
+TimeStep result equals the reference's CPU path), so the FP64 context is the
+validated FP64 solution at sizes the CPU cannot reach in reasonable time
+(8192^2 needs about 60 GB in the reference's own data structures).  Both runs
+start from the same initial fields and take the same number of steps; each
+computes its own CFL dt (FP64 reductions in both), so the FP32 run also
+accumulates a slightly different simulated time, which is reported.
+
+    python -m hyperflow_amd.accuracy [C1 C2 C3 C4 C5]
+
+prints one JSON object per configuration.
+"""
+from __future__ import annotations
+
+import json
+import sys
+from typing import Dict, List, Sequence
+
+import numpy as np
+
+from . import case as C
+from . import native
+
+VARS = ("rho", "rho_u", "rho_v", "E")
+
+
+def _run(cases, fields, steps, precision, graph, dt_mode="compat"):
+    s = native.Solver(cases, precision=precision, dt_mode=dt_mode)
+    try:
+        s.upload_all(fields)
+        s.run(steps, graph=graph)
+        s.synchronize()
+        out = s.download_all()
+        hist = s.dt_history()
+    finally:
+        s.close()
+    return out, hist
+
+
+def error_norms(a: Sequence[np.ndarray], b: Sequence[np.ndarray], g: int) -> Dict[str, dict]:
+    """Relative L1 (sum |a-b| / sum |b|) and relative L-inf (max |a-b| / max |b|)
+    over the interior cells of every block, per conserved variable."""
+    out = {}
+    for k, name in enumerate(VARS):
+        d1 = r1 = 0.0
+        dinf = rinf = 0.0
+        for x, y in zip(a, b):
+            xi = x[k, g:-g, g:-g]
+            yi = y[k, g:-g, g:-g]
+            diff = np.abs(xi - yi)
+            d1 += float(np.sum(diff))
+            r1 += float(np.sum(np.abs(yi)))
+            dinf = max(dinf, float(np.max(diff)))
+            rinf = max(rinf, float(np.max(np.abs(yi))))
+        out[name] = {"rel_l1": d1 / r1 if r1 else 0.0, "rel_linf": dinf / rinf if rinf else 0.0}
+    return out
+
+
+def compare(cases, fields, steps: int, graph: bool = False, dt_mode: str = "compat") -> dict:
+    """FP32 against FP64 on the same initial fields and step count."""
+    if isinstance(cases, C.Case):
+        cases = [cases]
+    out64, h64 = _run(cases, fields, steps, "fp64", graph, dt_mode)
+    out32, h32 = _run(cases, fields, steps, "fp32", graph, dt_mode)
+    g = cases[0].ghost
+    res = {"steps": steps, "errors": error_norms(out32, out64, g),
+           "finite": bool(all(np.all(np.isfinite(x[:, g:-g, g:-g])) for x in out32))}
+    if len(cases) == 1:
+        # simulated time of problem 0: the step dts are every cps-th TimeStep result
+        cps = 1 if cases[0].scheme == "Godunov" else 1 + len(cases[0].blocks)
+        t64 = float(np.sum(h64[::cps]))
+        t32 = float(np.sum(h32[::cps]))
+        res["t_fp64"] = t64
+        res["t_fp32"] = t32
+        res["t_rel_diff"] = abs(t32 - t64) / t64 if t64 else 0.0
+    return res
+
+
+def config(name: str):
+    """(description, cases, fields, steps, graph) of a benchmark configuration."""
+    if name == "C1":
+        c = C.godunov_quadrants(0)
+        return ("Godunov+HLLC 400^2, CFL 0.8, 100 steps, seed 0", [c], c.initial_fields(), 100, False)
+    if name == "C2":
+        c = C.muscl_step_channel(1024)
+        return ("MUSCL-Hancock+MinBee 8 x 1024^2 step channel, compat dt, 500 steps", [c], c.initial_fields(),
+                500, False)
+    if name == "C3":
+        c = C.muscl_quadrants(0, 8192)
+        return ("MUSCL-Hancock+VanLeer 8192^2, perturbed quadrants seed 0, 200 steps", [c],
+                C.perturbed_fields(c, 0), 200, False)
+    if name == "C4":
+        cases = [C.quad_case(nx, ny, 1000 + k, scheme="MUSCLHancock", limiter="VanLeer", cfl=0.5)
+                 for k, (nx, ny) in enumerate(C.batched_sizes(512, 0))]
+        return ("512 batched MUSCL-Hancock+VanLeer problems, sides U[64,512], 200 steps, HIP graph", cases,
+                [c.initial_fields()[0] for c in cases], 200, True)
+    if name == "C5":
+        c = C.muscl_quadrants(0, 1024)
+        return ("MUSCL-Hancock+VanLeer 1024^2 (the learned correction's coarse grid), perturbed quadrants "
+                "seed 0, 200 steps", [c], C.perturbed_fields(c, 0), 200, True)
+    raise ValueError(f"unknown configuration {name}")
+
+
+def table(names: Sequence[str] = ("C1", "C2", "C3", "C4", "C5"), log=None) -> List[dict]:
+    rows = []
+    for name in names:
+        desc, cases, fields, steps, graph = config(name)
+        r = compare(cases, fields, steps, graph=graph)
+        r.update({"config": name, "workload": desc})
+        if log:
+            log(json.dumps(r))
+        rows.append(r)
+        del fields
+    return rows
+
+
+if __name__ == "__main__":
+    table(sys.argv[1:] or ("C1", "C2", "C3", "C4", "C5"), log=lambda m: print(m, flush=True))

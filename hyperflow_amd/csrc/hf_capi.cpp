@@ -12,7 +12,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -34,6 +36,15 @@ int hf_fail(int code, const std::string& msg)
     g_err = msg;
     return code;
 }
+
+namespace {
+std::atomic<bool> g_exiting{false};
+// Registered when the library is loaded, i.e. after the HIP runtime it links
+// against: atexit handlers run in reverse order, so this one runs first.
+const int g_exit_hook = std::atexit([] { g_exiting.store(true); });
+}  // namespace
+
+bool hf_process_exiting() { return g_exiting.load(); }
 
 namespace {
 
@@ -251,7 +262,11 @@ HfStepArgs<R> step_args(hf_ctx* c, int item0, int nitems, const void* A, const v
     a.G = (const R*)G;
     a.O = (R*)O;
     a.nprob = c->np;
-    a.queue = c->d_queue;
+    a.nmain = nitems;
+    a.sync = c->d_sync;
+    a.rounds = c->d_rounds;
+    a.nrounds = 0;
+    a.pending = c->d_pending;
     a.imax_fix = c->d_imax_fix;
     a.fixed = c->d_fixed;
     a.skip = c->d_skip;
@@ -261,6 +276,7 @@ HfStepArgs<R> step_args(hf_ctx* c, int item0, int nitems, const void* A, const v
     a.gmax = c->gmax(c->cur);
     a.calls = c->d_calls;
     a.hist = c->d_hist;
+    a.hist_base = c->hist_base;
     a.hist_cap = c->hist_cap;
     a.dt_mode = c->dt_mode;
     a.gamma = c->gamma;
@@ -290,36 +306,38 @@ int launch_update_t(hf_ctx* c, HfStepArgs<R>& a, bool resid, bool timed)
 }
 
 // One persistent update launch over items [item0, item0+nitems), taking
-// tickets from queue slot qslot (zeroed before the launch runs).
-int launch_update(hf_ctx* c, int item0, int nitems, const void* A, const void* G, void* O, bool resid,
-                  bool fixed, double dt_step, double dt_pred, bool timed, int qslot, int fix_lb)
+// tickets with the sync counters of slot `slot` (zeroed before the launch
+// runs).  With rounds, the items after the first nmain are the fix-up rounds.
+int launch_update(hf_ctx* c, int item0, int nitems, int nmain, bool rounds, const void* A, const void* G, void* O,
+                  bool resid, bool fixed, double dt_step, double dt_pred, bool timed, int slot)
 {
     if (c->prec == HF_FP64) {
         HfStepArgs<double> a = step_args<double>(c, item0, nitems, A, G, O);
         a.use_fixed = fixed;
         a.fixed_step = dt_step;
         a.fixed_pred = dt_pred;
-        a.queue = c->d_queue + qslot;
-        a.fix_lb = fix_lb;
+        a.sync = c->d_sync + slot * HF_SYNC_WORDS;
+        a.nmain = nmain;
+        a.nrounds = rounds ? c->nrounds : 0;
         return launch_update_t<double>(c, a, resid, timed);
     }
     HfStepArgs<float> a = step_args<float>(c, item0, nitems, A, G, O);
     a.use_fixed = fixed;
     a.fixed_step = dt_step;
     a.fixed_pred = dt_pred;
-    a.queue = c->d_queue + qslot;
-    a.fix_lb = fix_lb;
+    a.sync = c->d_sync + slot * HF_SYNC_WORDS;
+    a.nmain = nmain;
+    a.nrounds = rounds ? c->nrounds : 0;
     return launch_update_t<float>(c, a, resid, timed);
 }
 
-// A launch outside the device-resident loop (class-level API): its queue
-// slot is cleared from the host first.
+// A launch outside the device-resident loop (class-level API): its sync
+// counters are cleared from the host first.
 int launch_update_host(hf_ctx* c, int item0, int nitems, const void* A, const void* G, void* O, bool resid,
                        double dt_step, double dt_pred, bool timed)
 {
-    const int slot = c->maxnb;
-    HF_HIP(hipMemsetAsync(c->d_queue + slot, 0, sizeof(uint32_t), c->stream));
-    return launch_update(c, item0, nitems, A, G, O, resid, true, dt_step, dt_pred, timed, slot, 0);
+    HF_HIP(hipMemsetAsync(c->d_sync + HF_SYNC_WORDS, 0, sizeof(uint32_t) * HF_SYNC_WORDS, c->stream));
+    return launch_update(c, item0, nitems, nitems, false, A, G, O, resid, true, dt_step, dt_pred, timed, 1);
 }
 
 int launch_ghost(hf_ctx* c, const HfGhost* tab, int n, const void* S, void* D, unsigned long long* gmax,
@@ -424,8 +442,10 @@ int enqueue_steps(hf_ctx* c, int nsteps, bool timed, bool fixed = false, double 
         bk.clear0 = c->gmax(k ^ 1);
         bk.clear1 = c->imax(k ^ 1);
         bk.nclear = 2 * c->nb;
-        bk.queue = c->d_queue;
-        bk.nqueue = c->maxnb;
+        bk.queue = c->d_sync;
+        bk.nqueue = HF_SYNC_WORDS;
+        bk.pending = c->d_pending;
+        bk.npending = c->npending;
         bk.nblocks = c->nb;
         bk.imax_prev = c->imax(k);
         bk.imax_fix = c->d_imax_fix;
@@ -434,13 +454,10 @@ int enqueue_steps(hf_ctx* c, int nsteps, bool timed, bool fixed = false, double 
         bk.skip_next = c->d_skip_next;
         int rc = launch_ghost(c, c->d_fill, c->nfill, A, B, c->gmax(k), bk);
         if (rc) return rc;
-        rc = launch_update(c, c->main_begin, c->main_items, A, B, B, false, fixed, dt, dt, timed, 0, 0);
+        // the main list and, with the CFL dt, the fix-up rounds in one launch
+        const int n = fixed ? c->main_items : c->main_items + c->fix_items;
+        rc = launch_update(c, c->main_begin, n, c->main_items, !fixed, A, B, B, false, fixed, dt, dt, timed, 0);
         if (rc) return rc;
-        for (size_t r = 0; !fixed && r < c->fix_rounds.size(); r++) {
-            rc = launch_update(c, c->fix_rounds[r].first, c->fix_rounds[r].second, A, B, B, false, false, 0.0, 0.0,
-                               timed, (int)r + 1, (int)r + 1);
-            if (rc) return rc;
-        }
         c->cur ^= 1;
     }
     c->cur = c0;
@@ -493,23 +510,23 @@ void destroy_graphs(hf_ctx* c)
     c->graphs.clear();
 }
 
-}  // namespace
-
-// =================================================================== C ABI ==
-
-extern "C" {
-
-const char* hf_last_error(void) { return g_err.c_str(); }
-
-const char* hf_version(void) { return "hyperflow_amd 0.1 gfx950"; }
-
-int hf_create(const hf_case_desc* desc, hf_ctx** out)
+// First item of each fix-up round relative to the main list's first item,
+// [1..nrounds], and the end of the last round at [nrounds + 1].
+std::vector<int32_t> round_table(const hf_ctx& c)
 {
-    if (!out) return fail(HF_EINVAL, "hf_create: null output pointer");
-    *out = nullptr;
-    int rc = validate(desc);
-    if (rc) return rc;
-    std::unique_ptr<hf_ctx> c(new hf_ctx());
+    std::vector<int32_t> rt(c.nrounds + 2, 0);
+    for (int l = 1; l <= c.nrounds; l++) rt[l] = c.fix_rounds[l - 1].first - c.main_begin;
+    rt[c.nrounds + 1] = c.main_items + c.fix_items;
+    return rt;
+}
+
+// The host-side plan of a case: block records and their layout in a state
+// buffer, problems, the work items of the update launches (main launch and
+// fix-up launches) and the resident-wave count they are sized for.  With
+// dense, blocks are stored back to back without row padding, the layout of
+// the arrays the C ABI exchanges.
+int build_plan(hf_ctx* c, const hf_case_desc* desc, bool dense)
+{
     c->scheme = desc->scheme;
     c->limiter = desc->limiter;
     c->prec = desc->precision;
@@ -523,6 +540,7 @@ int hf_create(const hf_case_desc* desc, hf_ctx** out)
     c->np = c->bdesc.back().problem + 1;
 
     // block layout: planes [var][ty][pitch], pitch a multiple of 32 elements
+    // (dense: pitch = nx+2g and blocks back to back, the layout of the C ABI's arrays)
     int64_t off = 0;
     for (int b = 0; b < c->nb; b++) {
         const hf_block_desc& D = c->bdesc[b];
@@ -533,13 +551,13 @@ int hf_create(const hf_case_desc* desc, hf_ctx** out)
         B.g = c->g;
         B.tx = D.nx + 2 * c->g;
         B.ty = D.ny + 2 * c->g;
-        B.pitch = (int32_t)round_up(B.tx, 32);
+        B.pitch = dense ? B.tx : (int32_t)round_up(B.tx, 32);
         B.plane = (int64_t)B.pitch * B.ty;
         B.off = off;
         B.problem = D.problem;
         B.dx = (D.x_right - D.x_left) / D.nx;     // MeshBlock::calculate_dx (mesh_block.cpp:227-230)
         B.dy = (D.y_top - D.y_bottom) / D.ny;
-        off = round_up(off + 4 * B.plane, 64);
+        off = dense ? off + 4 * B.plane : round_up(off + 4 * B.plane, 64);
         c->blocks.push_back(B);
         c->resid_elems = std::max<int64_t>(c->resid_elems, 4 * B.plane);
     }
@@ -650,6 +668,33 @@ int hf_create(const hf_case_desc* desc, hf_ctx** out)
                 if (P.nb > lb) add_rows(P.first + lb, 0, c->blocks[P.first + lb].ny, std::min(rows, fr), true);
             c->fix_rounds.emplace_back(first, (int)c->items.size() - first);
         }
+    // the fix-up rounds follow the main list in the item array
+    c->nrounds = (int)c->fix_rounds.size();
+    c->fix_items = (int)c->items.size() - (c->main_begin + c->main_items);
+    c->npending = std::max(1, (c->fix_items + 31) / 32);
+
+    return HF_OK;
+}
+
+}  // namespace
+
+// =================================================================== C ABI ==
+
+extern "C" {
+
+const char* hf_last_error(void) { return g_err.c_str(); }
+
+const char* hf_version(void) { return "hyperflow_amd 0.1 gfx950"; }
+
+int hf_create(const hf_case_desc* desc, hf_ctx** out)
+{
+    if (!out) return fail(HF_EINVAL, "hf_create: null output pointer");
+    *out = nullptr;
+    int rc = validate(desc);
+    if (rc) return rc;
+    std::unique_ptr<hf_ctx> c(new hf_ctx());
+    rc = build_plan(c.get(), desc, false);
+    if (rc) return rc;
 
     std::vector<HfGhost> fill, copy;
     build_tables(*c, fill, copy);
@@ -711,8 +756,15 @@ int hf_create(const hf_case_desc* desc, hf_ctx** out)
         HF_HIPC(hipMalloc(p, sizeof(uint32_t) * C->nb));
         HF_HIPC(hipMemset(*p, 0, sizeof(uint32_t) * C->nb));
     }
-    HF_HIPC(hipMalloc(&C->d_queue, sizeof(uint32_t) * (C->maxnb + 1)));
-    HF_HIPC(hipMemset(C->d_queue, 0, sizeof(uint32_t) * (C->maxnb + 1)));
+    HF_HIPC(hipMalloc(&C->d_sync, sizeof(uint32_t) * 2 * HF_SYNC_WORDS));
+    HF_HIPC(hipMemset(C->d_sync, 0, sizeof(uint32_t) * 2 * HF_SYNC_WORDS));
+    {
+        const std::vector<int32_t> rt = round_table(*C);
+        HF_HIPC(hipMalloc(&C->d_rounds, sizeof(int32_t) * rt.size()));
+        HF_HIPC(hipMemcpy(C->d_rounds, rt.data(), sizeof(int32_t) * rt.size(), hipMemcpyHostToDevice));
+    }
+    HF_HIPC(hipMalloc(&C->d_pending, sizeof(uint32_t) * C->npending));
+    HF_HIPC(hipMemset(C->d_pending, 0, sizeof(uint32_t) * C->npending));
     HF_HIPC(hipMalloc(&C->d_calls, sizeof(uint32_t) * C->np));
     HF_HIPC(hipMemset(C->d_calls, 0, sizeof(uint32_t) * C->np));
     HF_HIPC(hipHostMalloc((void**)&C->h_calls_pinned, sizeof(uint32_t) * C->np, hipHostMallocDefault));
@@ -730,6 +782,11 @@ int hf_create(const hf_case_desc* desc, hf_ctx** out)
 void hf_destroy(hf_ctx* c)
 {
     if (!c) return;
+    (void)g_exit_hook;
+    if (hf_process_exiting()) {   // see hf_process_exiting: host memory only
+        delete c;
+        return;
+    }
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     destroy_graphs(c);
     for (auto& e : c->ev_pending) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
@@ -737,7 +794,8 @@ void hf_destroy(hf_ctx* c)
     for (void* p : {c->buf[0], c->buf[1], (void*)c->d_blocks, (void*)c->d_probs, (void*)c->d_items, (void*)c->d_fill,
                     (void*)c->d_copy, c->d_consts, (void*)c->d_imax, (void*)c->d_gmax, (void*)c->d_scan,
                     (void*)c->d_calls, (void*)c->d_hist, (void*)c->d_part, c->d_resid, (void*)c->d_imax_fix,
-                    (void*)c->d_fixed, (void*)c->d_queue, (void*)c->d_skip, (void*)c->d_skip_next})
+                    (void*)c->d_fixed, (void*)c->d_sync, (void*)c->d_rounds, (void*)c->d_pending, (void*)c->d_skip,
+                    (void*)c->d_skip_next})
         if (p) (void)hipFree(p);
     if (c->h_calls_pinned) (void)hipHostFree(c->h_calls_pinned);
     if (c->calls_copied) (void)hipEventDestroy(c->calls_copied);

@@ -55,19 +55,29 @@ struct HfStepArgs {
     unsigned long long* imax_new;        // [nblocks][2] max wave speed of the new interior
     const unsigned long long* gmax;      // [nblocks][2] max wave speed of the ghosts
     const uint32_t* calls;               // TimeStep calls per problem (already bumped for this step)
-    double* hist;                        // TimeStep results of problem 0 by call index
+    double* hist;                        // TimeStep results of problem 0: call k goes to hist[k - 1 - hist_base]
+    int64_t hist_base;
     int32_t hist_cap;
     int32_t dt_mode;                     // 0 compat, 1 uniform
     double gamma, cfl;
     int32_t use_fixed;                   // 1: dt_step/dt_pred below instead of the device partials
-    int32_t fix_lb;                      // >0: fix-up launch for the fix_lb-th block of each problem
+    int32_t nmain;                       // items [0, nmain) form the main list, the rest the fix-up rounds
     double fixed_step, fixed_pred;
-    uint32_t* queue;                     // work-queue counter of this launch (zero at launch)
-    unsigned long long* imax_fix;        // [nblocks][2] maxima of blocks recomputed by a fix-up launch
-    uint32_t* fixed;                     // [nblocks] block was recomputed by a fix-up launch this step
-    uint32_t* skip;                      // [nblocks] main launch leaves the block to its fix-up launch
-    uint32_t* skip_next;                 // [nblocks] the fix-up launch's verdict for the next step
+    uint32_t* sync;                      // HF_SYNC_WORDS counters of this launch (zero at launch)
+    const int32_t* rounds;               // rounds[l]: first item of fix-up round l (1..nrounds); rounds[nrounds+1] = nitems
+    int32_t nrounds, pad1;
+    uint32_t* pending;                   // bitmap over the fix-up items left to the launch's last wave
+    unsigned long long* imax_fix;        // [nblocks][2] maxima of blocks recomputed in a fix-up round
+    uint32_t* fixed;                     // [nblocks] block was recomputed in a fix-up round this step
+    uint32_t* skip;                      // [nblocks] the main list leaves the block to its fix-up round
+    uint32_t* skip_next;                 // [nblocks] the fix-up verdict for the next step
 };
+
+// Counters of one update launch: the work-queue ticket counter, the number of
+// completed items (bit 31: the abort flag of the bounded waits) and the
+// number of waves that have left.
+enum { HF_SYNC_QUEUE = 0, HF_SYNC_DONE = 1, HF_SYNC_EXITED = 2, HF_SYNC_WORDS = 4 };
+#define HF_SYNC_ABORT_BIT 0x80000000u
 
 // Step bookkeeping done by the first workgroup of a ghost-fill launch.
 struct GhostBook {
@@ -78,8 +88,10 @@ struct GhostBook {
     unsigned long long* clear0;   // arrays to zero (next step's partials)
     unsigned long long* clear1;
     int32_t nclear;               // elements in each
-    uint32_t* queue;              // work-queue counters of this step's update launches
+    uint32_t* queue;              // sync counters of this step's update launch (HF_SYNC_WORDS)
     int32_t nqueue;
+    uint32_t* pending;            // pending bitmap of this step's update launch
+    int32_t npending;             // words
     int32_t nblocks;
     // fold the previous step's fix-ups into its maxima, then clear them
     unsigned long long* imax_prev;   // maxima of the state this step starts from

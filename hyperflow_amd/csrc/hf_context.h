@@ -15,6 +15,11 @@
 
 // Records msg as the calling thread's last error and returns code.
 int hf_fail(int code, const std::string& msg);
+// True once the process has started exiting (an atexit handler of this
+// library, which runs before the HIP runtime's own teardown).  Destroy calls
+// made after that release host memory only: the HIP runtime may already be
+// shutting down, and the driver reclaims the device memory with the process.
+bool hf_process_exiting();
 
 #define HF_HIP(call)                                                                         \
     do {                                                                                     \
@@ -54,7 +59,10 @@ struct hf_ctx {
     unsigned long long* d_scan = nullptr;   // [nb][2] scan results + [nb][2] scratch maxima
     unsigned long long* d_imax_fix = nullptr;   // [nb][2]
     uint32_t* d_fixed = nullptr;                // [nb]
-    uint32_t* d_queue = nullptr;                // [maxnb + 1] work-queue counters
+    uint32_t* d_sync = nullptr;                 // [2][HF_SYNC_WORDS]: the step loop's launch, class-level launches
+    int32_t* d_rounds = nullptr;                // [nrounds + 2] first item of each fix-up round (main-list relative)
+    uint32_t* d_pending = nullptr;              // [npending] bitmap over the fix-up items
+    int nrounds = 0, fix_items = 0, npending = 1;
     uint32_t* d_skip = nullptr;                 // [nb] block left to its fix-up launch this step
     uint32_t* d_skip_next = nullptr;            // [nb]
     uint32_t* d_calls = nullptr;
@@ -62,6 +70,7 @@ struct hf_ctx {
     hipEvent_t calls_copied = nullptr;   // the last upload of h_calls_pinned has been read
     double* d_hist = nullptr;
     int hist_cap = 1 << 20;
+    int64_t hist_base = 0;          // d_hist[k - 1 - hist_base] holds TimeStep call k of problem 0
     double* d_part = nullptr;
     int part_wg = 256;
     void* d_resid = nullptr;

@@ -50,24 +50,64 @@ __device__ __forceinline__ double shfl_dn1(double x)
 __device__ __forceinline__ float shfl_up1(float x) { return __int_as_float(dpp_up(__float_as_int(x))); }
 __device__ __forceinline__ float shfl_dn1(float x) { return __int_as_float(dpp_dn(__float_as_int(x))); }
 
-// Work items of a persistent launch: wave w first takes item w (no atomic,
-// so a launch does not open with every wave hitting one counter), then
-// tickets from the queue counter, offset by the number of waves.  Every lane
-// executes the atomic (only lane 0 adds 1) so that no lane-divergent region
-// surrounds the loop's exit test; lane 0's old value is the wave's ticket.  A
-// lane-0-only branch here let the compiler's control-flow structurizer keep
-// lanes 1..63 in the loop after lane 0 left it.
+// Work items of a persistent launch: wave w first takes item w for w below
+// nstatic (no atomic, so a launch does not open with every wave hitting one
+// counter), then tickets from the queue counter, offset by nstatic.  Every
+// lane executes the atomic (only lane 0 adds 1) so that no lane-divergent
+// region surrounds the loop's exit test; lane 0's old value is the wave's
+// ticket.  A lane-0-only branch here let the compiler's control-flow
+// structurizer keep lanes 1..63 in the loop after lane 0 left it.
 __device__ __forceinline__ unsigned total_waves() { return gridDim.x * (blockDim.x >> 6); }
-__device__ __forceinline__ unsigned first_ticket()
+__device__ __forceinline__ unsigned wave_id()
 {
     // wave-uniform, but derived from threadIdx: say so, or the whole item loop
     // is compiled as divergent (vector loads of the item and block records)
     return __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
 }
-__device__ __forceinline__ unsigned next_ticket(uint32_t* queue, int lane)
+__device__ __forceinline__ unsigned next_ticket(uint32_t* sync, int lane, unsigned nstatic)
 {
-    const unsigned t = atomicAdd(queue, lane == 0 ? 1u : 0u);
-    return __builtin_amdgcn_readfirstlane(t) + total_waves();
+    const unsigned t = atomicAdd(&sync[HF_SYNC_QUEUE], lane == 0 ? 1u : 0u);
+    return __builtin_amdgcn_readfirstlane(t) + nstatic;
+}
+__device__ __forceinline__ unsigned first_ticket(uint32_t* sync, int lane, unsigned nstatic)
+{
+    const unsigned w = wave_id();
+    return w < nstatic ? w : next_ticket(sync, lane, nstatic);
+}
+
+// One item of the launch is complete: its stores and maxima atomics are made
+// visible at device scope before the count that waiting waves poll.
+__device__ __forceinline__ void item_done(uint32_t* sync, int lane, unsigned n = 1)
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_fetch_add(&sync[HF_SYNC_DONE], lane == 0 ? n : 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Bounded wait until `target` items of the launch are complete (every item
+// before position `target` of the queue: items are handed out in order and an
+// item that waits never completes before the items it waits for).  The wait
+// gives up after WAIT_TICKS of the 100 MHz real-time counter, or at once when
+// another wave has given up, and then returns false with the abort bit set:
+// the caller leaves its item to the launch's last wave.  One load per poll
+// (count and abort bit share a word), about 0.85 us apart.
+constexpr unsigned long long WAIT_TICKS = 1000000ull;   // 10 ms
+__device__ __forceinline__ bool wait_done(uint32_t* sync, unsigned target)
+{
+    const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+    for (;;) {
+        const unsigned d = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(&sync[HF_SYNC_DONE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        if (d & HF_SYNC_ABORT_BIT) return false;
+        if (d >= target) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            return true;
+        }
+        if (__builtin_amdgcn_s_memrealtime() - t0 > WAIT_TICKS) {
+            __hip_atomic_fetch_or(&sync[HF_SYNC_DONE], HF_SYNC_ABORT_BIT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(32);
+    }
 }
 
 // A wave-reduced value as a wave-uniform (scalar) bit pattern.
@@ -183,10 +223,7 @@ __device__ __forceinline__ void muscl_item(const HfStepArgs<typename M::R>& a, c
 
         // calculate_spatial_update (muscl_hancock.cpp:364-371) and (dt*L)+U
         R Ln[4], Un[4];
-        for (int k = 0; k < 4; k++) {
-            Ln[k] = rdx * (Fw[k] - Fe[k]) + rdy * (Gs[k] - Gn[k]);
-            Un[k] = dtR * Ln[k] + uc[k];
-        }
+        update_cell<M>(rdx, rdy, dtR, Fw, Fe, Gs, Gn, uc, Ln, Un);
         if (out) {
             R* o = Ob + (int64_t)y * pitch + xraw;
             for (int k = 0; k < 4; k++) o[k * plane] = RESID ? Ln[k] : Un[k];
@@ -293,10 +330,7 @@ __device__ __forceinline__ void muscl_item_t(const HfStepArgs<typename M::R>& a,
 
         // calculate_spatial_update (muscl_hancock.cpp:364-371) and (dt*L)+U
         R Lu[4], Un[4];
-        for (int k = 0; k < 4; k++) {
-            Lu[k] = rdx * (Fw[k] - Fe[k]) + rdy * (Gs[k] - Gn[k]);
-            Un[k] = dtR * Lu[k] + uc[k];
-        }
+        update_cell<M>(rdx, rdy, dtR, Fw, Fe, Gs, Gn, uc, Lu, Un);
         if (out) {
             R* o = Ob + (int64_t)yraw * pitch + x;
             for (int k = 0; k < 4; k++) o[k * plane] = RESID ? Lu[k] : Un[k];
@@ -313,79 +347,175 @@ __device__ __forceinline__ void muscl_item_t(const HfStepArgs<typename M::R>& a,
 #undef COL
 }
 
+// The fix-up verdict of the lb-th block of problem p: the step dt, the
+// speculative predictor dt the main list used (every block at U^n) and the
+// reference's (blocks before this one at their final U^{n+1}); the block must
+// be (re)computed in its fix-up round if the main list skipped it or the two
+// predictor dts differ.  Also records the verdict for the next step.
+template <class R>
+__device__ __forceinline__ bool fix_verdict(const HfStepArgs<R>& a, int p, const HfProbDev& pr, int lb, double& dt_step,
+                                            double& dt_pred, long long& kstep, long long& kpred)
+{
+    double spec;
+    step_dts_fix<R>(a, p, pr, lb, dt_step, spec, dt_pred, kstep, kpred);
+    const int b = pr.first + lb;
+    a.skip_next[b] = spec != dt_pred ? 1u : 0u;   // the same value from every wave
+    return a.skip[b] != 0u || spec != dt_pred;
+}
+
+// Fix-up rounds from round l on, while they are empty: round l holds the
+// lb = l-th block of every problem with more than l blocks, and its verdicts
+// need the blocks before it final, which holds while every earlier round was
+// empty.  The queue counter jumps past each empty round (its unissued items
+// are counted complete).  Returns the first round with work, or nrounds + 1.
+template <class R>
+__device__ int first_busy_round(const HfStepArgs<R>& a, int l, int lane, unsigned nstatic)
+{
+    for (; l <= a.nrounds; l++) {
+        bool need = false;
+        for (int p = 0; p < a.nprob; p++) {
+            const HfProbDev pr = a.probs[p];
+            if (pr.nb <= l) continue;
+            double st, pd;
+            long long ks, kp;
+            need = fix_verdict<R>(a, p, pr, l, st, pd, ks, kp) || need;
+        }
+        if (need) return l;
+        const unsigned end = (unsigned)a.rounds[l + 1] - nstatic;
+        const unsigned old = __builtin_amdgcn_readfirstlane(atomicMax(&a.sync[HF_SYNC_QUEUE], end));
+        if (old < end) item_done(a.sync, lane, end - old);
+    }
+    return a.nrounds + 1;
+}
+
+// Which fix-up round an item position belongs to (rounds are few).
+template <class R>
+__device__ __forceinline__ int round_of(const HfStepArgs<R>& a, unsigned idx)
+{
+    int l = 1;
+    while (l < a.nrounds && idx >= (unsigned)a.rounds[l + 1]) l++;
+    return l;
+}
+
 // Persistent MUSCL-Hancock launch: waves take work items from a queue until
-// it is empty.  The main launch of a step computes every block with the
-// speculative predictor dt (every block at U^n), except blocks marked skip[b]
-// (their speculation failed in the previous step).  A fix-up launch
-// (fix_lb > 0) then computes the fix_lb-th block of each problem with the
-// reference's dt -- earlier blocks at their final U^{n+1} -- when the block
-// was skipped or its speculative dt differs, so results equal the
-// reference's sequential sweep bit for bit.  The fix-up launch also records
-// whether the block will need it again (skip_next).
+// it is empty.  The queue holds the main list (every block with the
+// speculative predictor dt, all blocks at U^n, except blocks marked skip[b]:
+// their speculation failed in the previous step), then the fix-up rounds of
+// the reference-compatible dt (muscl_hancock.cpp:246 with the sequential block
+// loop of simulation.cpp:106-110): round l recomputes the l-th block of each
+// problem with the reference's predictor dt, earlier blocks at their final
+// U^{n+1}, wherever the main list skipped it or its speculative dt differs.
+// An item of round l first waits (bounded) until every earlier item of the
+// queue is complete.  Empty rounds are skipped as a whole.  A wait that gives
+// up leaves its item to the launch's last wave, which computes every such
+// item in queue order once all other waves have left: no wait there, so the
+// launch always ends, and the results are the same bit for bit.
+template <class M, int LIM, bool RESID>
+__device__ __forceinline__ void muscl_go(const HfStepArgs<typename M::R>& a, const HfItem& it, int lane, double dt_step,
+                                         double dt_pred, unsigned long long* imax_out)
+{
+    if (!RESID && item_transposed(it))
+        muscl_item_t<M, LIM, RESID>(a, it, lane, dt_step, dt_pred, imax_out);
+    else
+        muscl_item<M, LIM, RESID>(a, it, lane, dt_step, dt_pred, imax_out);
+}
+
+template <class R>
+__device__ __forceinline__ void record_dts(const HfStepArgs<R>& a, const HfItem& it, const HfBlockDev& bd, bool step,
+                                           double dt_step, double dt_pred, long long kstep, long long kpred)
+{
+    if (bd.problem == 0 && it.x0 == 0 && it.y0 == bd.g) {   // same value from every lane
+        const long long hs = kstep - a.hist_base, hp = kpred - a.hist_base;
+        if (step && bd.lb == 0 && hs >= 1 && hs <= a.hist_cap) a.hist[hs - 1] = dt_step;
+        if (hp >= 1 && hp <= a.hist_cap) a.hist[hp - 1] = dt_pred;
+    }
+}
+
 template <class M, int LIM, bool RESID>
 __global__ __launch_bounds__(256) void k_muscl(const HfStepArgs<typename M::R> a)
 {
     typedef typename M::R R;
     const int lane = threadIdx.x & 63;
-    const bool fix = a.fix_lb > 0;
-    int cached = -1;          // block whose dts (and fix verdict) this wave has computed
+    const unsigned nstatic = min(total_waves(), (unsigned)a.nmain);
+    // the block whose dts this wave has computed: speculative ones for the main
+    // list (c_fix false) or the fix-up verdict and exact ones (c_fix true)
+    int cached = -1;
+    bool c_fix = false, c_need = false;
     double c_step = 0.0, c_pred = 0.0;
     long long c_kstep = 0, c_kpred = 0;
-    bool c_need = false;
-    if (fix) {
-        // leave at once unless some problem's fix_lb-th block needs computing;
-        // the last block examined stays cached for the item loop
-        bool any = false;
-        for (int p = 0; p < a.nprob; p++) {
-            const HfProbDev pr = a.probs[p];
-            if (pr.nb <= a.fix_lb) continue;
-            const int b = pr.first + a.fix_lb;
-            double spec;
-            step_dts_fix<R>(a, p, pr, a.fix_lb, c_step, spec, c_pred, c_kstep, c_kpred);
-            a.skip_next[b] = spec != c_pred ? 1u : 0u;   // same value from every wave
-            c_need = a.skip[b] || spec != c_pred;
-            cached = b;
-            any = any || c_need;
-        }
-        if (!any) return;
-    }
-    for (unsigned idx = first_ticket();; idx = next_ticket(a.queue, lane)) {
+    int r_from = -1, r_busy = 0;   // first_busy_round(r_from) as this wave last computed it
+    for (unsigned idx = first_ticket(a.sync, lane, nstatic);; idx = next_ticket(a.sync, lane, nstatic)) {
         if (idx >= (unsigned)a.nitems) break;
         const HfItem it = a.items[a.item0 + idx];
-        double dt_step, dt_pred;
         if (a.use_fixed) {
-            dt_step = a.fixed_step;
-            dt_pred = a.fixed_pred;
-        } else {
-            if (!fix && a.skip[it.block]) continue;   // computed exactly by its fix-up launch
-            const HfBlockDev bd = a.blocks[it.block];
-            if (it.block != cached) {
-                cached = it.block;
-                if (fix) {
-                    double spec;
-                    const HfProbDev pr = a.probs[bd.problem];
-                    step_dts_fix<R>(a, bd.problem, pr, bd.lb, c_step, spec, c_pred, c_kstep, c_kpred);
-                    c_need = a.skip[it.block] || spec != c_pred;
-                } else {
+            muscl_go<M, LIM, RESID>(a, it, lane, a.fixed_step, a.fixed_pred, a.imax_new);
+            item_done(a.sync, lane);
+            continue;
+        }
+        const HfBlockDev bd = a.blocks[it.block];
+        if (idx < (unsigned)a.nmain) {
+            if (!a.skip[it.block]) {   // a skipped block is computed exactly in its fix-up round
+                if (it.block != cached || c_fix) {
+                    cached = it.block;
+                    c_fix = false;
                     step_dts<R>(a, it.block, bd, true, DT_SPEC, c_step, c_pred, c_kstep, c_kpred);
                 }
+                record_dts<R>(a, it, bd, true, c_step, c_pred, c_kstep, c_kpred);
+                muscl_go<M, LIM, RESID>(a, it, lane, c_step, c_pred, a.imax_new);
             }
-            dt_step = c_step;
-            dt_pred = c_pred;
-            const long long kstep = c_kstep, kpred = c_kpred;
-            if (fix) {
-                if (!c_need) continue;
+            item_done(a.sync, lane);
+            continue;
+        }
+        // fix-up round item
+        const int l = round_of<R>(a, idx);
+        if (l != r_from) {
+            if (!wait_done(a.sync, (unsigned)a.rounds[l])) {
+                atomicOr(&a.pending[(idx - a.nmain) >> 5], 1u << ((idx - a.nmain) & 31));
+                continue;   // left to the last wave, not counted complete
+            }
+            r_from = l;
+            r_busy = first_busy_round<R>(a, l, lane, nstatic);
+        }
+        if (r_busy == l) {
+            if (it.block != cached || !c_fix) {
+                cached = it.block;
+                c_fix = true;
+                const HfProbDev pr = a.probs[bd.problem];
+                c_need = fix_verdict<R>(a, bd.problem, pr, bd.lb, c_step, c_pred, c_kstep, c_kpred);
+            }
+            if (c_need) {
                 a.fixed[it.block] = 1u;
-            }
-            if (bd.problem == 0 && it.x0 == 0 && it.y0 == bd.g) {   // same value from every lane
-                if (!fix && bd.lb == 0 && kstep >= 1 && kstep <= a.hist_cap) a.hist[kstep - 1] = dt_step;
-                if (kpred >= 1 && kpred <= a.hist_cap) a.hist[kpred - 1] = dt_pred;
+                record_dts<R>(a, it, bd, false, c_step, c_pred, c_kstep, c_kpred);
+                muscl_go<M, LIM, RESID>(a, it, lane, c_step, c_pred, a.imax_fix);
             }
         }
-        if (!RESID && item_transposed(it))
-            muscl_item_t<M, LIM, RESID>(a, it, lane, dt_step, dt_pred, fix ? a.imax_fix : a.imax_new);
-        else
-            muscl_item<M, LIM, RESID>(a, it, lane, dt_step, dt_pred, fix ? a.imax_fix : a.imax_new);
+        item_done(a.sync, lane);
+    }
+    // The last wave to leave computes the items whose wait gave up, in queue
+    // order; every other wave has left, so all it reads is final.
+    const unsigned left = __builtin_amdgcn_readfirstlane(atomicAdd(&a.sync[HF_SYNC_EXITED], lane == 0 ? 1u : 0u));
+    if (left != total_waves() - 1) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (!(__builtin_amdgcn_readfirstlane(a.sync[HF_SYNC_DONE]) & HF_SYNC_ABORT_BIT)) return;
+    const unsigned nfix = (unsigned)(a.nitems - a.nmain);
+    for (unsigned w = 0; w * 32 < nfix; w++) {
+        const unsigned bits = __builtin_amdgcn_readfirstlane(a.pending[w]);
+        for (unsigned k = 0; k < 32; k++) {
+            if (!((bits >> k) & 1u)) continue;
+            const unsigned idx = a.nmain + w * 32 + k;
+            const HfItem it = a.items[a.item0 + idx];
+            const HfBlockDev bd = a.blocks[it.block];
+            const HfProbDev pr = a.probs[bd.problem];
+            double dt_step, dt_pred;
+            long long kstep, kpred;
+            if (fix_verdict<R>(a, bd.problem, pr, bd.lb, dt_step, dt_pred, kstep, kpred)) {
+                a.fixed[it.block] = 1u;
+                record_dts<R>(a, it, bd, false, dt_step, dt_pred, kstep, kpred);
+                muscl_go<M, LIM, RESID>(a, it, lane, dt_step, dt_pred, a.imax_fix);
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");   // its maxima before the next verdict
+            }
+        }
+        a.pending[w] = 0u;
     }
 }
 
@@ -446,10 +576,7 @@ __device__ __forceinline__ void godunov_item(const HfStepArgs<typename M::R>& a,
 
         const R U[4] = {sc.U0, sc.U1, sc.U2, sc.U3};
         R Ln[4], Un[4];
-        for (int k = 0; k < 4; k++) {
-            Ln[k] = rdx * (Fw[k] - Fe[k]) + rdy * (Gs[k] - Gn[k]);
-            Un[k] = dtR * Ln[k] + U[k];
-        }
+        update_cell<M>(rdx, rdy, dtR, Fw, Fe, Gs, Gn, U, Ln, Un);
         if (out) {
             R* o = Ob + (int64_t)y * pitch + xraw;
             for (int k = 0; k < 4; k++) o[k * plane] = RESID ? Ln[k] : Un[k];
@@ -473,7 +600,8 @@ __global__ __launch_bounds__(256) void k_godunov(const HfStepArgs<typename M::R>
 {
     typedef typename M::R R;
     const int lane = threadIdx.x & 63;
-    for (unsigned idx = first_ticket();; idx = next_ticket(a.queue, lane)) {
+    const unsigned nstatic = min(total_waves(), (unsigned)a.nitems);
+    for (unsigned idx = first_ticket(a.sync, lane, nstatic);; idx = next_ticket(a.sync, lane, nstatic)) {
         if (idx >= (unsigned)a.nitems) break;
         const HfItem it = a.items[a.item0 + idx];
         double dt_step, dt_pred;
@@ -483,9 +611,9 @@ __global__ __launch_bounds__(256) void k_godunov(const HfStepArgs<typename M::R>
             const HfBlockDev bd = a.blocks[it.block];
             long long kstep, kpred;
             step_dts<R>(a, it.block, bd, false, DT_SPEC, dt_step, dt_pred, kstep, kpred);
-            if (bd.problem == 0 && bd.lb == 0 && it.x0 == 0 && it.y0 == bd.g && kstep >= 1 &&
-                kstep <= a.hist_cap)
-                a.hist[kstep - 1] = dt_step;
+            const long long hs = kstep - a.hist_base;
+            if (bd.problem == 0 && bd.lb == 0 && it.x0 == 0 && it.y0 == bd.g && hs >= 1 && hs <= a.hist_cap)
+                a.hist[hs - 1] = dt_step;
         }
         godunov_item<M, RESID>(a, it, lane, dt_step, a.imax_new);
     }
@@ -517,6 +645,7 @@ __global__ __launch_bounds__(256) void k_ghost(const HfGhost* __restrict__ tab, 
             if (book.clear1) book.clear1[i] = 0ull;
         }
         for (int i = threadIdx.x; i < book.nqueue; i += blockDim.x) book.queue[i] = 0u;
+        for (int i = threadIdx.x; i < book.npending; i += blockDim.x) book.pending[i] = 0u;
         for (int b = threadIdx.x; b < book.nblocks && book.fixed; b += blockDim.x) {
             if (book.fixed[b]) {
                 book.imax_prev[2 * b] = book.imax_fix[2 * b];

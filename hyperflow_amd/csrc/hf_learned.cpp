@@ -309,6 +309,10 @@ int hf_net_create(const hf_net_desc* desc, hf_net** out)
 void hf_net_destroy(hf_net* n)
 {
     if (!n) return;
+    if (hf_process_exiting()) {   // host memory only once the process is exiting
+        delete n;
+        return;
+    }
     (void)hipDeviceSynchronize();
     free_images(n);
     (void)hipFree(n->params);

@@ -23,6 +23,7 @@ struct MathF64 {
     static __device__ __forceinline__ double sqrt(double x) { return __builtin_sqrt(x); }
     static __device__ __forceinline__ double fabs(double x) { return __builtin_fabs(x); }
     static constexpr double tol = 1e-6;
+    static constexpr bool fast = false;   // the reference's expressions, in its order
 };
 
 // Bulk FP32: hardware reciprocal and square root (about 1 ulp each).
@@ -32,6 +33,7 @@ struct MathF32 {
     static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
     static __device__ __forceinline__ float fabs(float x) { return __builtin_fabsf(x); }
     static constexpr float tol = 1e-6f;
+    static constexpr bool fast = true;    // the FP32 formulation of namespace f32 below
 };
 
 // std::min / std::max (NaN handling of the reference:
@@ -71,15 +73,206 @@ __device__ __forceinline__ typename M::R sound(typename M::R gamma, typename M::
     return M::sqrt(M::div(gamma * p, rho));
 }
 
+// ------------------------------------------------------------ FP32 bulk --
+// The FP32 path owes no bit-for-bit contract, so it evaluates the same
+// scheme in fewer operations (results agree with the FP64 path to FP32
+// rounding):
+//  * one reciprocal of rho per state, shared by u, v and the sound speed;
+//  * the state's own total energy E where the reference re-derives it from
+//    the primitives (rho (q^2/2 + p/(rho (gamma-1))) equals E up to rounding);
+//  * rho u and rho v are the conserved momenta themselves in the fluxes;
+//  * the limiters with one reciprocal (the ratio and xi(r) folded into one
+//    quotient), HLLC's star state with two, the flux of the upwind side only;
+//  * multiply-adds written as explicit fmaf.  The build disables contraction,
+//    so no other operation is fused and every copy of this code (warm-up and
+//    loop body of a sweep) rounds the same way: results do not depend on how
+//    the rows are split into work items.
+namespace f32 {
+
+__device__ __forceinline__ float fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// the limiter ratio's tolerance clamp (muscl_hancock.cpp:8-11,78-94):
+// |d| <= 1e-6 becomes +-1e-6 (zero becomes +1e-6)
+__device__ __forceinline__ float clamp_tol(float d)
+{
+    const float tol = 1e-6f;
+    return (__builtin_fabsf(d) <= tol) ? ((d >= 0.0f) ? tol : -tol) : d;
+}
+
+// u, v, p of a conserved state; ri = 1/rho
+__device__ __forceinline__ void prims(float U0, float U1, float U2, float U3, float g1, float& ri, float& u, float& v,
+                                      float& p)
+{
+    ri = rcp(U0);
+    u = U1 * ri;
+    v = U2 * ri;
+    p = g1 * fma(-0.5f, fma(U1, u, U2 * v), U3);
+}
+
+__device__ __forceinline__ State<float> make_state(const float U[4], float gamma, float g1)
+{
+    State<float> s;
+    s.U0 = U[0]; s.U1 = U[1]; s.U2 = U[2]; s.U3 = U[3];
+    float ri;
+    prims(U[0], U[1], U[2], U[3], g1, ri, s.u, s.v, s.p);
+    s.E = U[3];
+    s.a = __builtin_amdgcn_sqrtf(gamma * s.p * ri);
+    return s;
+}
+
+// Euler flux normal to the face: X (x-face) or y-face, E = the state's own energy
+template <bool X>
+__device__ __forceinline__ void flux(float U0, float U1, float U2, float U3, float u, float v, float p, float F[4])
+{
+    (void)U0;
+    if (X) {
+        F[0] = U1;
+        F[1] = fma(U1, u, p);
+        F[2] = U1 * v;
+        F[3] = u * (U3 + p);
+    } else {
+        F[0] = U2;
+        F[1] = U2 * u;
+        F[2] = fma(U2, v, p);
+        F[3] = v * (U3 + p);
+    }
+}
+
+// HLLC with the Davis wave speeds, Toro's star state rewritten with
+// m_K = rho_K (S_K - q_K):  S* = (p_R - p_L + m_L q_L - m_R q_R) / (m_L - m_R),
+// U*_K = [m_K, m_K S*, m_K t_K, (S_K - q_K) E_K + (S* - q_K)(m_K S* + p_K)] / (S_K - S*).
+// The branch order of the reference: 0 <= S_L full left flux, then the left
+// and right star fluxes, 0 >= S_R full right flux, otherwise (NaN) zero.
+template <bool X>
+__device__ __forceinline__ void hllc(const State<float>& L, const State<float>& Rs, float F[4])
+{
+    const float qL = X ? L.u : L.v, qR = X ? Rs.u : Rs.v;
+    const float S_L = smin(qL - L.a, qR - Rs.a);
+    const float S_R = smax(qL + L.a, qR + Rs.a);
+    const float mL = L.U0 * (S_L - qL), mR = Rs.U0 * (S_R - qR);
+    const float Ss = fma(mL, qL, fma(-mR, qR, Rs.p - L.p)) * rcp(mL - mR);
+
+    // Branch choice of the reference for finite speeds: S_L >= 0 takes the
+    // full left flux; otherwise S* >= 0 the left star flux; otherwise the
+    // right side, its star flux if S_R >= 0, else the full right flux.  So
+    // K = R exactly when S_L < 0 and S* < 0, and the star term's weight
+    // S_K (zero for a full flux) is min(S_L, 0) on the left, max(S_R, 0) on
+    // the right.  A NaN speed gives the zero flux, as in the reference.
+    const bool useR = (S_L < 0.0f) & (Ss < 0.0f);
+    const float wL = (S_L < 0.0f) ? S_L : 0.0f, wR = (S_R > 0.0f) ? S_R : 0.0f;
+    const float chk = S_L + S_R + Ss;
+    const bool none = !(chk == chk);
+    const float dK = useR ? Rs.U0 : L.U0;
+    const float U1K = useR ? Rs.U1 : L.U1;
+    const float U2K = useR ? Rs.U2 : L.U2;
+    const float EK = useR ? Rs.U3 : L.U3;
+    const float uK = useR ? Rs.u : L.u;
+    const float vK = useR ? Rs.v : L.v;
+    const float pK = useR ? Rs.p : L.p;
+    const float SK = useR ? S_R : S_L;
+    const float mK = useR ? mR : mL;
+    const float qK = X ? uK : vK, tK = X ? vK : uK;
+    const float UnK = X ? U1K : U2K, UtK = X ? U2K : U1K;
+
+    float FK[4];
+    flux<X>(dK, U1K, U2K, EK, uK, vK, pK, FK);
+    const float rS = rcp(SK - Ss);
+    const float us0 = mK * rS;
+    const float us3 = rS * fma(SK - qK, EK, (Ss - qK) * fma(mK, Ss, pK));
+    const float w = useR ? wR : wL;
+    const bool star = w != 0.0f;   // a full flux never sees the star state (it may not be finite)
+    const float d0 = us0 - dK;
+    const float dn = fma(us0, Ss, -UnK);
+    const float dt = fma(us0, tK, -UtK);
+    const float d3 = us3 - EK;
+    const float F0 = star ? fma(w, d0, FK[0]) : FK[0];
+    const float Fn = star ? fma(w, X ? dn : dt, FK[1]) : FK[1];
+    const float Ft = star ? fma(w, X ? dt : dn, FK[2]) : FK[2];
+    const float F3 = star ? fma(w, d3, FK[3]) : FK[3];
+    F[0] = none ? 0.0f : F0;
+    F[1] = none ? 0.0f : Fn;
+    F[2] = none ? 0.0f : Ft;
+    F[3] = none ? 0.0f : F3;
+}
+
+// Limited slope xi(r) * (dl + dr) / 2 with r from the clamped deltas
+// (muscl_hancock.cpp:67-206), one reciprocal.  For r > 0 (same signs):
+//  MinBee  r <= 1: xi = r              -> (dl+dr)/2 * dlc / drc
+//          r >  1: xi = 2/(1+r)        -> (dl+dr) * drc / (drc + dlc)
+//  VanLeer xi = 2 min(r, 1)/(1+r)      -> (dl+dr) * m / (dlc + drc), m = dlc or drc, whichever is
+//          smaller in magnitude (both have the sign of drc)
+template <int LIM>
+__device__ __forceinline__ float slope(float um, float uc, float up)
+{
+    const float dl = uc - um, dr = up - uc;
+    const float dlc = clamp_tol(dl), drc = clamp_tol(dr);
+    // r > 0: the clamped deltas have one sign (they are never zero, and their
+    // product stays far above the FP32 underflow)
+    const bool same = dlc * drc > 0.0f;
+    const bool le1 = __builtin_fabsf(dlc) <= __builtin_fabsf(drc);   // |r| <= 1
+    float num, den;
+    if (LIM == 0) {
+        num = le1 ? 0.5f * dlc : drc;
+        den = le1 ? drc : drc + dlc;
+    } else {
+        num = le1 ? dlc : drc;
+        den = dlc + drc;
+    }
+    const float s = (dl + dr) * num * rcp(den);
+    return same ? s : 0.0f;
+}
+
+// One direction of MUSCL-Hancock reconstruction of a cell: face states
+// U -+ s/2 and the split half-step predictor face += h (F(lo) - F(hi)).
+template <int LIM, bool X>
+__device__ __forceinline__ void reconstruct(const float um[4], const float uc[4], const float up[4], float h, float g1,
+                                            float lo[4], float hi[4])
+{
+    for (int k = 0; k < 4; k++) {
+        const float s = slope<LIM>(um[k], uc[k], up[k]);
+        lo[k] = fma(-0.5f, s, uc[k]);
+        hi[k] = fma(0.5f, s, uc[k]);
+    }
+    float ri, ul, vl, pl, ur, vr, pr;
+    prims(lo[0], lo[1], lo[2], lo[3], g1, ri, ul, vl, pl);
+    prims(hi[0], hi[1], hi[2], hi[3], g1, ri, ur, vr, pr);
+    float Fl[4], Fr[4];
+    flux<X>(lo[0], lo[1], lo[2], lo[3], ul, vl, pl, Fl);
+    flux<X>(hi[0], hi[1], hi[2], hi[3], ur, vr, pr, Fr);
+    for (int k = 0; k < 4; k++) {
+        const float d = Fl[k] - Fr[k];
+        lo[k] = fma(h, d, lo[k]);
+        hi[k] = fma(h, d, hi[k]);
+    }
+}
+
+// |u| + c and |v| + c of a state (TimeStep's per-cell terms)
+__device__ __forceinline__ void wave_speeds(float U0, float U1, float U2, float U3, float gamma, float g1, double& Dx,
+                                            double& Dy)
+{
+    float ri, u, v, p;
+    prims(U0, U1, U2, U3, g1, ri, u, v, p);
+    const float c = __builtin_amdgcn_sqrtf(gamma * p * ri);
+    Dx = (double)(__builtin_fabsf(u) + c);
+    Dy = (double)(__builtin_fabsf(v) + c);
+}
+
+}  // namespace f32
+
 template <class M>
 __device__ __forceinline__ State<typename M::R> make_state(const typename M::R U[4], typename M::R gamma,
                                                            typename M::R g1)
 {
-    State<typename M::R> s;
-    s.U0 = U[0]; s.U1 = U[1]; s.U2 = U[2]; s.U3 = U[3];
-    prims<M>(U[0], U[1], U[2], U[3], g1, s.u, s.v, s.p, s.E);
-    s.a = sound<M>(gamma, s.p, U[0]);
-    return s;
+    if constexpr (M::fast) {
+        return f32::make_state(U, gamma, g1);
+    } else {
+        State<typename M::R> s;
+        s.U0 = U[0]; s.U1 = U[1]; s.U2 = U[2]; s.U3 = U[3];
+        prims<M>(U[0], U[1], U[2], U[3], g1, s.u, s.v, s.p, s.E);
+        s.a = sound<M>(gamma, s.p, U[0]);
+        return s;
+    }
 }
 
 // cons_flux_x / cons_flux_y (euler_ideal_gas.cpp:113-149) from precomputed
@@ -108,6 +301,10 @@ __device__ __forceinline__ void hllc(const State<typename M::R>& L, const State<
                                      typename M::R F[4])
 {
     typedef typename M::R R;
+    if constexpr (M::fast) {
+        f32::hllc<X>(L, Rs, F);
+        return;
+    }
     const R qL = X ? L.u : L.v, qR = X ? Rs.u : Rs.v;
     const R S_L = smin(qL - L.a, qR - Rs.a);
     const R S_R = smax(qL + L.a, qR + Rs.a);
@@ -212,6 +409,10 @@ __device__ __forceinline__ void reconstruct(const typename M::R um[4], const typ
                                             typename M::R lo[4], typename M::R hi[4])
 {
     typedef typename M::R R;
+    if constexpr (M::fast) {
+        f32::reconstruct<LIM, X>(um, uc, up, h, g1, lo, hi);
+        return;
+    }
     for (int k = 0; k < 4; k++) {
         const R s = slope<M, LIM>(um[k], uc[k], up[k]);
         lo[k] = uc[k] - R(0.5) * s;
@@ -230,6 +431,26 @@ __device__ __forceinline__ void reconstruct(const typename M::R um[4], const typ
     }
 }
 
+// calculate_spatial_update (muscl_hancock.cpp:364-371, godunov.cpp:97-111)
+// and the forward Euler step (dt*L)+U (forward_euler.cpp:27-28): the
+// reference's order in FP64, multiply-adds in FP32.
+template <class M>
+__device__ __forceinline__ void update_cell(typename M::R rdx, typename M::R rdy, typename M::R dt,
+                                            const typename M::R Fw[4], const typename M::R Fe[4],
+                                            const typename M::R Gs[4], const typename M::R Gn[4],
+                                            const typename M::R U[4], typename M::R L[4], typename M::R Un[4])
+{
+    for (int k = 0; k < 4; k++) {
+        if constexpr (M::fast) {
+            L[k] = f32::fma(rdx, Fw[k] - Fe[k], rdy * (Gs[k] - Gn[k]));
+            Un[k] = f32::fma(dt, L[k], U[k]);
+        } else {
+            L[k] = rdx * (Fw[k] - Fe[k]) + rdy * (Gs[k] - Gn[k]);
+            Un[k] = dt * L[k] + U[k];
+        }
+    }
+}
+
 // Bits of a non-negative double; their unsigned order is the numeric order,
 // which lets wave-speed maxima use 64-bit integer atomics.
 __device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
@@ -243,6 +464,10 @@ __device__ __forceinline__ void wave_speeds(typename M::R U0, typename M::R U1, 
                                             double& Dx, double& Dy)
 {
     typedef typename M::R R;
+    if constexpr (M::fast) {
+        f32::wave_speeds(U0, U1, U2, U3, gamma, g1, Dx, Dy);
+        return;
+    }
     const R u = M::div(U1, U0);
     const R v = M::div(U2, U0);
     const R p = g1 * (U3 - R(0.5) * U0 * (u * u + v * v));

@@ -282,6 +282,10 @@ int hf_uns_create(const hf_uns_desc* d, hf_uns** out)
 void hf_uns_destroy(hf_uns* u)
 {
     if (!u) return;
+    if (hf_process_exiting()) {   // host memory only once the process is exiting
+        delete u;
+        return;
+    }
     if (u->stream) (void)hipStreamSynchronize(u->stream);
     for (auto* list : {&u->ev_pending, &u->ev_free})
         for (auto& e : *list) {

@@ -146,6 +146,9 @@ def make_pairs(seeds, K: int = 4, n_corr: int = 50, net=None, log=print):
                 coarse.state_to_device(0, P[i])
                 fine.run_fixed(2 * K, dtf, graph=True)
                 fine.restrict2(0, T[i])
+                # P[i] was written on the coarse solver's stream, T[i] on the
+                # fine one's: both must be complete before torch's stream reads
+                coarse.synchronize()
                 fine.synchronize()
                 if net is not None and not bool(torch.isfinite(P[i]).all()):
                     valid[i] = False          # the rollout broke down: restart it from the target

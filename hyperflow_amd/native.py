@@ -28,7 +28,8 @@ HF_RUN_GRAPH = 1
 
 EXPORTS = ("hf_create", "hf_destroy", "hf_ghost_cells", "hf_upload", "hf_download", "hf_apply_bcs",
            "hf_time_step", "hf_scheme_residual", "hf_advance", "hf_run", "hf_synchronize", "hf_dt_history",
-           "hf_calls", "hf_profile", "hf_kernel_stats", "hf_conservation", "hf_last_error", "hf_version")
+           "hf_calls", "hf_profile", "hf_kernel_stats", "hf_conservation", "hf_last_error",
+           "hf_version")
 # include/hyperflow_learned.h
 EXPORTS_LEARNED = ("hf_net_create", "hf_net_destroy", "hf_net_set_stream", "hf_net_set_desc", "hf_net_param_count",
                    "hf_net_set_params", "hf_net_get_params", "hf_net_reset_optimizer", "hf_net_forward",
@@ -143,6 +144,29 @@ def block_descs(cases: Sequence[Case]):
     return arr
 
 
+def case_desc(cases, precision: str = "fp64", dt_mode: str = "compat", tile_rows: int = 0):
+    """hf_case_desc for one or more cases (case k = problem k).  Returns the
+    descriptor and the block array it points into (keep both alive)."""
+    if isinstance(cases, Case):
+        cases = [cases]
+    c0 = cases[0]
+    for c in cases[1:]:
+        if (c.scheme, c.limiter, c.gamma, c.cfl) != (c0.scheme, c0.limiter, c0.gamma, c0.cfl):
+            raise ValueError("all problems of one solver share scheme, limiter, gamma and cfl")
+    descs = block_descs(cases)
+    d = hf_case_desc()
+    d.scheme = SCHEMES[c0.scheme]
+    d.limiter = LIMITERS.get(c0.limiter or "MinBee", 0)
+    d.precision = HF_FP64 if precision == "fp64" else HF_FP32
+    d.dt_mode = HF_DT_COMPAT if dt_mode == "compat" else HF_DT_UNIFORM
+    d.gamma = c0.gamma
+    d.cfl = c0.cfl
+    d.nblocks = len(descs)
+    d.tile_rows = tile_rows
+    d.blocks = C.cast(descs, C.POINTER(hf_block_desc))
+    return d, descs
+
+
 class Solver:
     """One device context: a mesh of blocks (one or more independent problems)
     plus the scheme, advanced by the gfx950 kernels."""
@@ -156,17 +180,7 @@ class Solver:
             if (c.scheme, c.limiter, c.gamma, c.cfl) != (c0.scheme, c0.limiter, c0.gamma, c0.cfl):
                 raise ValueError("all problems of one solver share scheme, limiter, gamma and cfl")
         self.blocks = [b for c in self.cases for b in c.blocks]
-        self._descs = block_descs(self.cases)
-        d = hf_case_desc()
-        d.scheme = SCHEMES[c0.scheme]
-        d.limiter = LIMITERS.get(c0.limiter or "MinBee", 0)
-        d.precision = HF_FP64 if precision == "fp64" else HF_FP32
-        d.dt_mode = HF_DT_COMPAT if dt_mode == "compat" else HF_DT_UNIFORM
-        d.gamma = c0.gamma
-        d.cfl = c0.cfl
-        d.nblocks = len(self._descs)
-        d.tile_rows = tile_rows
-        d.blocks = C.cast(self._descs, C.POINTER(hf_block_desc))
+        d, self._descs = case_desc(self.cases, precision, dt_mode, tile_rows)
         self._desc = d
         h = C.c_void_p()
         _check(lib().hf_create(C.byref(d), C.byref(h)), "hf_create")

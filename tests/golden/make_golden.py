@@ -9,6 +9,7 @@ fields, the fields after `nsteps` steps and every TimeStep result.  Only data
 is stored; no reference source.
 
     python tests/golden/make_golden.py --nonphysical  # cells with negative pressure, zero density, NaN
+    python tests/golden/make_golden.py --doublerp     # DoubleRiemannProblem ICs, Dirichlet in both variable kinds
     python tests/golden/make_golden.py --cli     # CSV frames of the reference program
     python tests/golden/make_golden.py --uns     # unstructured solver, small meshes
     python tests/golden/make_golden.py --uns-nonphysical  # the same with non-physical cells
@@ -74,6 +75,35 @@ def cases():
     return out
 
 
+def doublerp_cases():
+    """DoubleRiemannProblem initial conditions (doublerp.cpp:42-55, parsed at
+    parser.cpp:325-356): three x-bands of constant state.  One case with
+    "Primitive" variables whose left interface lies exactly on a cell
+    centroid (the `<=` comparison decides that column), one with
+    "Conserved" variables behind Dirichlet sides given in both variable kinds
+    (parser.cpp:441-453)."""
+    out = []
+    nx, ny, g = 40, 16, 2
+    dx = (1.0 - 0.0) / nx
+    x_on_centroid = (0.0 - (float(g) - 0.5) * dx) + 13 * dx     # MeshBlock::calculate_centroids
+    c = C.quad_case(nx, ny, 0, scheme="MUSCLHancock", limiter="MinBee", cfl=0.5, bc="Reflective")
+    c.initial_condition = {"type": "DoubleRiemannProblem", "variables": "Primitive",
+                           "x_lm_interface": x_on_centroid, "x_mr_interface": 0.7,
+                           "init_left_state": [1.0, 0.75, -0.2, 1.0], "init_middle_state": [0.125, 0.0, 0.3, 0.1],
+                           "init_right_state": [0.8, -0.5, 0.0, 0.6]}
+    out.append(("doublerp_primitive_minbee_40x16", c, None, (1, 60)))
+    c = C.quad_case(36, 20, 0, scheme="Godunov", cfl=0.8, bc="Reflective", extents=(-1.0, 2.0, 0.0, 1.5))
+    c.initial_condition = {"type": "DoubleRiemannProblem", "variables": "Conserved",
+                           "x_lm_interface": -0.25, "x_mr_interface": 1.1,
+                           "init_left_state": [1.0, 0.5, 0.1, 2.6], "init_middle_state": [0.5, -0.1, 0.0, 1.2],
+                           "init_right_state": [1.5, 0.0, 0.3, 3.9]}
+    b = c.blocks[0].boundaries
+    b["left"] = {"type": "Dirichlet", "variables": "Conserved", "flow_state": [1.2, 0.9, 0.0, 3.1]}
+    b["right"] = {"type": "Dirichlet", "variables": "Primitive", "flow_state": [0.9, -0.3, 0.05, 1.3]}
+    out.append(("doublerp_conserved_godunov_dirichlet_36x20", c, None, (1, 60)))
+    return out
+
+
 def nonphysical_cases():
     """Seeded cases with three non-physical cells (C.nonphysical_fields):
     negative pressure, zero density, a NaN momentum."""
@@ -91,7 +121,15 @@ def main(which=cases):
     if not hfo.have_reference():
         sys.exit("oracle/_ref/ref_harness is not built (needs the reference checkout)")
     for name, case, fields, steps in which():
-        init = fields if fields is not None else case.initial_fields()
+        if fields is not None:
+            init = fields
+        else:
+            # the reference's own initial fields (its IC classes, no steps);
+            # the tests compare hyperflow_amd's initial fields against them
+            init, _, _, _ = hfo.run_reference(case, 0)
+            mine = case.initial_fields()
+            if not all(np.array_equal(a, b) for a, b in zip(mine, init)):
+                print(f"warning: {name}: initial fields differ from the reference's")
         data = {"case_json": np.array(json.dumps(case.to_json())), "nblocks": np.array(len(case.blocks))}
         for b, U in enumerate(init):
             data[f"init_{b}"] = U
@@ -110,6 +148,8 @@ if __name__ == "__main__" and len(sys.argv) == 1:
     main()
 if __name__ == "__main__" and sys.argv[1:] == ["--nonphysical"]:
     main(nonphysical_cases)
+if __name__ == "__main__" and sys.argv[1:] == ["--doublerp"]:
+    main(doublerp_cases)
 
 
 def cli_goldens():
