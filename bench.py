@@ -234,11 +234,12 @@ def build_workload(name: str, n: int, precision: str, seed: int = 0):
             f"MUSCL-Hancock+MinBee+HLLC {precision.upper()}, 8 blocks of {n}x{n} step channel (Dirichlet inflow, "
             "reflective walls, transmissive outflow), CFL 0.5, reference-compatible dt")
     if name == "muscl-quadrants":
+        from hyperflow_amd.accuracy import C3_SEED
         n3 = 8192 if n == 1024 else n
-        case = C.muscl_quadrants(seed, n3)
-        return [case], C.perturbed_fields(case, seed), (
+        case = C.muscl_quadrants(C3_SEED, n3)
+        return [case], C.perturbed_fields(case, C3_SEED), (
             f"MUSCL-Hancock+VanLeer+HLLC {precision.upper()}, single block {n3}x{n3}, seeded quadrants with "
-            "sinusoidal density perturbation, CFL 0.5")
+            f"sinusoidal density perturbation (seed {C3_SEED}), CFL 0.5")
     if name == "batch":
         cases = [C.quad_case(nx, ny, 1000 + k, scheme="MUSCLHancock", limiter="VanLeer", cfl=0.5)
                  for k, (nx, ny) in enumerate(C.batched_sizes(512, seed))]
@@ -280,7 +281,7 @@ def main():
     ap.add_argument("--refine", type=int, default=8, help="unstructured: cells per coarse cell and direction")
     ap.add_argument("--n", type=int, default=1024, help="cells per block side (step-channel), grid side (muscl-quadrants)")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default=None)
-    ap.add_argument("--tile-rows", type=int, default=None, help="default: autotuned value if cached")
+    ap.add_argument("--tile-rows", type=int, default=None, help="default: the autotuned work plan of the class")
     ap.add_argument("--graph", action="store_true", help="capture the timed steps in a HIP graph")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -304,10 +305,12 @@ def main():
 
     precision = args.precision or ("fp32" if args.workload == "batch" else "fp64")
     cases, fields, workload_text = build_workload(args.workload, args.n, precision)
-    tile_rows = args.tile_rows if args.tile_rows is not None else autotune.cached_rows(cases, precision)
+    plan = autotune.cached_params(cases, precision)
+    if args.tile_rows is not None:
+        plan = {"tile_rows": args.tile_rows, "tail_rows": 0, "fix_rows": 0}
     graph = args.graph or args.workload == "batch"
 
-    solver = native.Solver(cases, precision=precision, tile_rows=tile_rows)
+    solver = native.Solver(cases, precision=precision, **plan)
     solver.upload_all(fields)
     solver.run(args.warmup, graph=graph)
     solver.synchronize()
@@ -356,8 +359,8 @@ def main():
                  if args.workload == "step-channel" else "synthetic (seeded quadrant states of the named workload)"),
         "config": {"workload": workload_text, "blocks": sum(len(c.blocks) for c in cases),
                    "interior_cells": cells, "scheme": cases[0].scheme, "limiter": cases[0].limiter,
-                   "riemann": "HLLC", "cfl": cases[0].cfl, "tile_rows": tile_rows or "default",
-                   "hip_graph": graph},
+                   "riemann": "HLLC", "cfl": cases[0].cfl,
+                   "work_plan": {k: (v or "default") for k, v in plan.items()}, "hip_graph": graph},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": kernel if launches else "whole step (graph)", "avg_launch_ms": round(avg_ms, 5),

@@ -25,6 +25,12 @@ from . import case as C
 from . import native
 
 VARS = ("rho", "rho_u", "rho_v", "E")
+# Seed of the perturbed-quadrant workloads.  Seed 0's data drive the FP64
+# solution itself into negative pressure within 50 steps (the scheme is not
+# positivity preserving near the vacuum that diverging quadrant velocities
+# open), after which two solutions of non-physical states are compared;
+# seed 1 stays physical over the 200 steps at 1024^2 and 8192^2.
+C3_SEED = 1
 
 
 def _run(cases, fields, steps, precision, graph, dt_mode="compat"):
@@ -89,18 +95,18 @@ def config(name: str):
         return ("MUSCL-Hancock+MinBee 8 x 1024^2 step channel, compat dt, 500 steps", [c], c.initial_fields(),
                 500, False)
     if name == "C3":
-        c = C.muscl_quadrants(0, 8192)
-        return ("MUSCL-Hancock+VanLeer 8192^2, perturbed quadrants seed 0, 200 steps", [c],
-                C.perturbed_fields(c, 0), 200, False)
+        c = C.muscl_quadrants(C3_SEED, 8192)
+        return (f"MUSCL-Hancock+VanLeer 8192^2, perturbed quadrants seed {C3_SEED}, 200 steps", [c],
+                C.perturbed_fields(c, C3_SEED), 200, False)
     if name == "C4":
         cases = [C.quad_case(nx, ny, 1000 + k, scheme="MUSCLHancock", limiter="VanLeer", cfl=0.5)
                  for k, (nx, ny) in enumerate(C.batched_sizes(512, 0))]
         return ("512 batched MUSCL-Hancock+VanLeer problems, sides U[64,512], 200 steps, HIP graph", cases,
                 [c.initial_fields()[0] for c in cases], 200, True)
     if name == "C5":
-        c = C.muscl_quadrants(0, 1024)
+        c = C.muscl_quadrants(C3_SEED, 1024)
         return ("MUSCL-Hancock+VanLeer 1024^2 (the learned correction's coarse grid), perturbed quadrants "
-                "seed 0, 200 steps", [c], C.perturbed_fields(c, 0), 200, True)
+                f"seed {C3_SEED}, 200 steps", [c], C.perturbed_fields(c, C3_SEED), 200, True)
     raise ValueError(f"unknown configuration {name}")
 
 

@@ -43,10 +43,12 @@ UNITS = [
 ]
 CLI = os.path.join(HERE, "bin", "hyperflow")
 CLIS = ((CLI, "host/hf_cli.cpp"), (os.path.join(HERE, "bin", "hyperflow_uns"), "host/hf_uns_cli.cpp"))
+# test driver of the reference-typed C++ surface (tests/cpp), linked like the CLIs
+TEST_DRIVERS = ((os.path.join(ROOT, "tests", "bin", "ref_loop"), os.path.join(ROOT, "tests", "cpp", "ref_loop.cpp")),)
 
 
 def _compile(unit, flags, hidden, outdir):
-    src = os.path.join(CSRC, unit)
+    src = unit if os.path.isabs(unit) else os.path.join(CSRC, unit)
     obj = os.path.join(outdir, os.path.basename(unit) + ".o")
     cmd = [HIPCC] + COMMON + (["-fvisibility=hidden"] if hidden else []) + flags + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
@@ -72,6 +74,15 @@ def build(verbose: bool = False) -> str:
     for exe, src in CLIS:
         cli_obj = _compile(src, ["-x", "c++", "-O2"], False, outdir)
         cmd = [HIPCC, "-o", exe + ".tmp", cli_obj, f"-L{HERE}", "-lhyperflow_amd", "-Wl,-rpath,$ORIGIN/.."]
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"{src}: link failed:\n" + r.stdout)
+        os.replace(exe + ".tmp", exe)
+    for exe, src in TEST_DRIVERS:
+        os.makedirs(os.path.dirname(exe), exist_ok=True)
+        obj = _compile(src, ["-x", "c++", "-O2"], False, outdir)
+        cmd = [HIPCC, "-o", exe + ".tmp", obj, f"-L{HERE}", "-lhyperflow_amd",
+               "-Wl,-rpath,$ORIGIN/../../hyperflow_amd"]
         r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"{src}: link failed:\n" + r.stdout)

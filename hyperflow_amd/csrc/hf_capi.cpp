@@ -83,7 +83,8 @@ int validate(const hf_case_desc* d)
     if (d->dt_mode != HF_DT_COMPAT && d->dt_mode != HF_DT_UNIFORM) return fail(HF_EINVAL, "case: unknown dt mode");
     if (!std::isfinite(d->gamma) || !std::isfinite(d->cfl) || d->cfl <= 0.0)
         return fail(HF_EINVAL, "case: gamma and cfl must be finite, cfl > 0");
-    if (d->tile_rows < 0) return fail(HF_EINVAL, "case: tile_rows must be >= 0");
+    if (d->tile_rows < 0 || d->tail_rows < 0 || d->fix_rows < 0)
+        return fail(HF_EINVAL, "case: tile_rows, tail_rows and fix_rows must be >= 0");
     const int g = d->scheme == HF_MUSCL_HANCOCK ? 2 : 1;
     int prev_problem = 0;
     for (int b = 0; b < d->nblocks; b++) {
@@ -639,7 +640,8 @@ int build_plan(hf_ctx* c, const hf_case_desc* desc, bool dense)
         for (int b = 0; b < c->nb; b++) work += (double)full_strips(b) * c->blocks[b].ny;
         const double f3 = std::min(1.0 / 16, 4.25 * (double)wave_slots / std::max(1.0, work));
         const double f2 = std::min(0.25, 17.0 * (double)wave_slots / std::max(1.0, work));
-        const int r2 = std::max(1, std::min(rows, std::max(8, rows / 2))), r3 = std::max(1, std::min(rows, 8));
+        const int tail = desc->tail_rows > 0 ? desc->tail_rows : 8;
+        const int r2 = std::max(1, std::min(rows, std::max(tail, rows / 2))), r3 = std::max(1, std::min(rows, tail));
         std::vector<int> y1(c->nb), y2(c->nb);
         for (int b = 0; b < c->nb; b++) {
             const int ny = c->blocks[b].ny;
@@ -662,7 +664,9 @@ int build_plan(hf_ctx* c, const hf_case_desc* desc, bool dense)
                     work += (int64_t)full_strips(P.first + lb) * c->blocks[P.first + lb].ny;
                 }
             const int64_t target = std::max<int64_t>(1, wave_slots * 95 / 100);
-            const int fr = (int)std::max<int64_t>(4, std::min<int64_t>(16, (work + target - 1) / target));
+            const int fr = desc->fix_rows > 0
+                               ? desc->fix_rows
+                               : (int)std::max<int64_t>(4, std::min<int64_t>(16, (work + target - 1) / target));
             const int first = (int)c->items.size();
             for (const auto& P : c->probs)
                 if (P.nb > lb) add_rows(P.first + lb, 0, c->blocks[P.first + lb].ny, std::min(rows, fr), true);

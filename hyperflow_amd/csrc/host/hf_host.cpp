@@ -64,6 +64,8 @@ public:
         c.cfl = ts.get_cfl();
         c.nblocks = (int32_t)d.size();
         c.tile_rows = opt.tile_rows;
+        c.tail_rows = opt.tail_rows;
+        c.fix_rows = opt.fix_rows;
         c.blocks = d.data();
         check(hf_create(&c, &ctx), "hf_create");
         for (size_t b = 0; b < blocks.size(); b++) {
@@ -75,7 +77,7 @@ public:
     ~Device()
     {
         for (auto& b : mesh.get_mesh_blocks()) {
-            b->get_fields();                 // keep the last state on the host
+            b->get_field_array();            // keep the last state on the host
             b->device = nullptr;
         }
         hf_destroy(ctx);
@@ -195,32 +197,79 @@ Vec1D MeshBlock::get_centroid(unsigned i, unsigned j) const
     return Vec1D{0.0, 0.0};
 }
 
-const std::vector<double>& MeshBlock::get_fields() const
+namespace {
+
+// fields[x][y][var] (the reference's Vec3D) from a flat [var][y][x] array
+Vec3D to_vec3d(const std::vector<double>& f, unsigned tx, unsigned ty)
+{
+    Vec3D out(tx, Vec2D(ty, Vec1D(4)));
+    for (unsigned i = 0; i < tx; i++)
+        for (unsigned j = 0; j < ty; j++)
+            for (unsigned v = 0; v < 4; v++) out[i][j][v] = f[((size_t)v * ty + j) * tx + i];
+    return out;
+}
+
+std::vector<double> from_vec3d(const Vec3D& F, unsigned tx, unsigned ty, const char* what)
+{
+    if (F.size() != tx) throw std::invalid_argument(std::string(what) + ": expected nx+2g columns");
+    std::vector<double> f((size_t)4 * tx * ty);
+    for (unsigned i = 0; i < tx; i++) {
+        if (F[i].size() != ty) throw std::invalid_argument(std::string(what) + ": expected ny+2g rows");
+        for (unsigned j = 0; j < ty; j++) {
+            if (F[i][j].size() != 4) throw std::invalid_argument(std::string(what) + ": expected 4 variables");
+            for (unsigned v = 0; v < 4; v++) f[((size_t)v * ty + j) * tx + i] = F[i][j][v];
+        }
+    }
+    return f;
+}
+
+}  // namespace
+
+const std::vector<double>& MeshBlock::get_field_array() const
 {
     if (device && !host_current) {
         device->download(index, fields);
         host_current = true;
+        fields3_current = false;
     }
     return fields;
 }
 
-void MeshBlock::set_fields(const std::vector<double>& f)
+void MeshBlock::set_field_array(const std::vector<double>& f)
 {
     if (f.size() != fields.size()) throw std::invalid_argument("set_fields: size must be 4*(nx+2g)*(ny+2g)");
     fields = f;
     host_current = true;
+    fields3_current = false;
     if (device) device->upload(index, fields);
+}
+
+const Vec3D& MeshBlock::get_fields() const
+{
+    const std::vector<double>& f = get_field_array();
+    if (!fields3_current) {
+        fields3 = to_vec3d(f, get_total_x_cells(), get_total_y_cells());
+        fields3_current = true;
+    }
+    return fields3;
+}
+
+void MeshBlock::set_fields(const Vec3D& F)
+{
+    set_field_array(from_vec3d(F, get_total_x_cells(), get_total_y_cells(), "set_fields"));
+    fields3 = F;
+    fields3_current = true;
 }
 
 double MeshBlock::get_flow_value(unsigned i, unsigned j, unsigned var) const
 {
-    const std::vector<double>& f = get_fields();
+    const std::vector<double>& f = get_field_array();
     return f[((size_t)var * get_total_y_cells() + j) * get_total_x_cells() + i];
 }
 
 void MeshBlock::initialise_field_values(const InitialCondition& ic)
 {
-    std::vector<double> f = get_fields();
+    std::vector<double> f = get_field_array();
     const unsigned tx = get_total_x_cells(), ty = get_total_y_cells();
     for (unsigned i = ghost_cells; i < tx - ghost_cells; i++)
         for (unsigned j = ghost_cells; j < ty - ghost_cells; j++) {
@@ -228,7 +277,7 @@ void MeshBlock::initialise_field_values(const InitialCondition& ic)
             const Vec1D v = ic(c[0], c[1]);
             for (unsigned k = 0; k < 4; k++) f[((size_t)k * ty + j) * tx + i] = v[k];
         }
-    set_fields(f);
+    set_field_array(f);
 }
 
 Mesh::Mesh(const std::vector<std::shared_ptr<MeshBlock> >& b,
@@ -273,7 +322,12 @@ GodunovScheme::GodunovScheme(const std::shared_ptr<Model>& m, const std::shared_
                              const std::shared_ptr<TimeStep>& t)
     : model(m), riemann(r), time_step(t) {}
 
-Vec1D GodunovScheme::operator()(const std::shared_ptr<MeshBlock>& block)
+Vec3D Scheme::operator()(const std::shared_ptr<MeshBlock>& block)
+{
+    return to_vec3d(residual(block), block->get_total_x_cells(), block->get_total_y_cells());
+}
+
+Vec1D GodunovScheme::residual(const std::shared_ptr<MeshBlock>& block)
 {
     if (!block->device) throw std::logic_error("Scheme: block is not bound to a device");
     return block->device->residual(block->index, block->fields.size());
@@ -286,13 +340,30 @@ MUSCLHancockScheme::MUSCLHancockScheme(const std::shared_ptr<Model>& m, const st
                                        const std::shared_ptr<Mesh>& msh)
     : model(m), riemann(r), time_step(t), limiter(l), mesh(msh) {}
 
-Vec1D MUSCLHancockScheme::operator()(const std::shared_ptr<MeshBlock>& block)
+Vec1D MUSCLHancockScheme::residual(const std::shared_ptr<MeshBlock>& block)
 {
     if (!block->device) throw std::logic_error("Scheme: block is not bound to a device");
     return block->device->residual(block->index, block->fields.size());
 }
 
 double MUSCLHancockScheme::calculate_time_step(const std::shared_ptr<Mesh>& m) { return (*time_step)(m); }
+
+Vec3D ForwardEulerODESolver::operator()(const std::function<Vec3D()>& func, const Vec3D& U, double dt)
+{
+    const Vec3D L = func();
+    if (L.size() != U.size()) throw std::invalid_argument("ODESolver: update and fields differ in shape");
+    Vec3D out(U.size());
+    for (size_t i = 0; i < U.size(); i++) {
+        if (L[i].size() != U[i].size()) throw std::invalid_argument("ODESolver: update and fields differ in shape");
+        out[i].resize(U[i].size());
+        for (size_t j = 0; j < U[i].size(); j++) {
+            if (L[i][j].size() != U[i][j].size()) throw std::invalid_argument("ODESolver: update and fields differ in shape");
+            out[i][j].resize(U[i][j].size());
+            for (size_t v = 0; v < U[i][j].size(); v++) out[i][j][v] = (dt * L[i][j][v]) + U[i][j][v];
+        }
+    }
+    return out;
+}
 
 void ForwardEulerODESolver::advance(const std::shared_ptr<Mesh>& mesh, double dt)
 {
@@ -321,7 +392,7 @@ void EulerIdealGasDataOutput::to_file(const std::shared_ptr<Mesh>& mesh, unsigne
     for (auto& blk : mesh->get_mesh_blocks()) {
         const MeshBlock& B = *blk;
         const unsigned g = B.get_ghost_cells(), tx = B.get_total_x_cells(), ty = B.get_total_y_cells();
-        const std::vector<double>& f = B.get_fields();
+        const std::vector<double>& f = B.get_field_array();
         for (unsigned i = g; i < tx - g; i++)
             for (unsigned j = g; j < ty - g; j++) {
                 const Vec1D c = B.get_centroid(i, j);
@@ -406,6 +477,8 @@ JSONParser::JSONParser(const hf_json::Value& cfg) : config(std::make_shared<hf_j
         if (o.has("precision")) options.precision = o["precision"].str() == "fp32" ? HF_FP32 : HF_FP64;
         if (o.has("dt_mode")) options.dt_mode = o["dt_mode"].str() == "uniform" ? HF_DT_UNIFORM : HF_DT_COMPAT;
         if (o.has("tile_rows")) options.tile_rows = o["tile_rows"].integer();
+        if (o.has("tail_rows")) options.tail_rows = o["tail_rows"].integer();
+        if (o.has("fix_rows")) options.fix_rows = o["fix_rows"].integer();
     }
 }
 

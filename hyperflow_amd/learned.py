@@ -63,21 +63,22 @@ def make_case(n: int, seed: int) -> C.Case:
 
 def solver(n: int, seed: int, fields=None) -> native.Solver:
     case = make_case(n, seed)
-    rows = autotune.cached_rows(case, "fp32", "uniform")
-    s = native.Solver(case, precision="fp32", dt_mode="uniform", tile_rows=rows)
+    plan = autotune.cached_params(case, "fp32", "uniform")
+    s = native.Solver(case, precision="fp32", dt_mode="uniform", **plan)
     if fields is not None:
         s.upload(0, fields)
     return s
 
 
 def ensure_tuned(resolutions, log=print) -> None:
-    """Autotune the tile rows of every resolution used (FP32, one block)."""
+    """Autotune the work plan of every resolution's class (FP32, one block)
+    that has no entry of its own yet."""
     for n in resolutions:
         case = make_case(n, 0)
-        if autotune.cached_rows(case, "fp32", "uniform"):
+        if autotune.has_class(case, "fp32", "uniform"):
             continue
-        r = autotune.tune(case, C.perturbed_fields(case, 0), precision="fp32", dt_mode="uniform")
-        log(f"tuned {n}^2: tile_rows {r['tile_rows']} ({r['ms_per_step']:.4f} ms/step)")
+        r = autotune.tune(case, C.perturbed_fields(case, 0), precision="fp32", dt_mode="uniform", label=f"learned-{n}")
+        log(f"tuned {n}^2: tile_rows {r['tile_rows']} tail_rows {r['tail_rows']} ({r['ms_per_step']:.4f} ms/step)")
 
 
 # ------------------------------------------------------------------ data --

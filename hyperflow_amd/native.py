@@ -46,7 +46,7 @@ class hf_block_desc(C.Structure):
 class hf_case_desc(C.Structure):
     _fields_ = [("scheme", C.c_int32), ("limiter", C.c_int32), ("precision", C.c_int32), ("dt_mode", C.c_int32),
                 ("gamma", C.c_double), ("cfl", C.c_double), ("nblocks", C.c_int32), ("tile_rows", C.c_int32),
-                ("blocks", C.POINTER(hf_block_desc))]
+                ("tail_rows", C.c_int32), ("fix_rows", C.c_int32), ("blocks", C.POINTER(hf_block_desc))]
 
 
 class hf_net_desc(C.Structure):
@@ -144,7 +144,8 @@ def block_descs(cases: Sequence[Case]):
     return arr
 
 
-def case_desc(cases, precision: str = "fp64", dt_mode: str = "compat", tile_rows: int = 0):
+def case_desc(cases, precision: str = "fp64", dt_mode: str = "compat", tile_rows: int = 0, tail_rows: int = 0,
+              fix_rows: int = 0):
     """hf_case_desc for one or more cases (case k = problem k).  Returns the
     descriptor and the block array it points into (keep both alive)."""
     if isinstance(cases, Case):
@@ -163,6 +164,8 @@ def case_desc(cases, precision: str = "fp64", dt_mode: str = "compat", tile_rows
     d.cfl = c0.cfl
     d.nblocks = len(descs)
     d.tile_rows = tile_rows
+    d.tail_rows = tail_rows
+    d.fix_rows = fix_rows
     d.blocks = C.cast(descs, C.POINTER(hf_block_desc))
     return d, descs
 
@@ -171,7 +174,8 @@ class Solver:
     """One device context: a mesh of blocks (one or more independent problems)
     plus the scheme, advanced by the gfx950 kernels."""
 
-    def __init__(self, cases, precision: str = "fp64", dt_mode: str = "compat", tile_rows: int = 0):
+    def __init__(self, cases, precision: str = "fp64", dt_mode: str = "compat", tile_rows: int = 0,
+                 tail_rows: int = 0, fix_rows: int = 0):
         if isinstance(cases, Case):
             cases = [cases]
         self.cases: List[Case] = list(cases)
@@ -180,7 +184,7 @@ class Solver:
             if (c.scheme, c.limiter, c.gamma, c.cfl) != (c0.scheme, c0.limiter, c0.gamma, c0.cfl):
                 raise ValueError("all problems of one solver share scheme, limiter, gamma and cfl")
         self.blocks = [b for c in self.cases for b in c.blocks]
-        d, self._descs = case_desc(self.cases, precision, dt_mode, tile_rows)
+        d, self._descs = case_desc(self.cases, precision, dt_mode, tile_rows, tail_rows, fix_rows)
         self._desc = d
         h = C.c_void_p()
         _check(lib().hf_create(C.byref(d), C.byref(h)), "hf_create")

@@ -76,7 +76,13 @@ typedef struct {
     double gamma;            /* EulerIdealGasModel(gamma), euler_ideal_gas.cpp:14-17 */
     double cfl;              /* TimeStep(model, cfl), timestep.cpp:14-21 */
     int32_t nblocks;
-    int32_t tile_rows;       /* rows per wave sweep; 0 = library default */
+    /* Work plan of the update launches (results do not depend on it; 0 = the
+     * library default): rows a wave sweeps per item of the main list, rows per
+     * item of the main list's closing phase, rows per item of the fix-up
+     * rounds of the compat predictor dt. */
+    int32_t tile_rows;
+    int32_t tail_rows;
+    int32_t fix_rows;
     const hf_block_desc* blocks;
 } hf_case_desc;
 

@@ -7,8 +7,13 @@
 // of hyperflow.h; these classes hold descriptions and host copies only.
 //
 // Differences a user of the reference sees:
-//  * fields are exchanged as flat vectors [var][y][x] (x fastest) instead of
-//    Vec3D fields[x][y][var];
+//  * Vec1D is std::vector<double> and Vec3D nested std::vectors, with the
+//    reference's [x][y][var] indexing; MeshBlock::get_fields/set_fields,
+//    Scheme::operator() and ODESolver::operator() take and return Vec3D as
+//    the reference's do (slow adapters: every call copies the block between
+//    host and device).  The fast paths exchange flat [var][y][x] arrays
+//    (x fastest, the C ABI's layout): MeshBlock::get_field_array /
+//    set_field_array, Scheme::residual and ODESolver::advance;
 //  * a Mesh is bound to a device context when a Simulation is constructed
 //    (or by Mesh::attach); Mesh::apply_bcs, TimeStep and Scheme calls before
 //    that throw std::logic_error;
@@ -17,6 +22,7 @@
 //    where nlohmann::json would throw.
 #pragma once
 
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
@@ -30,6 +36,8 @@ class Value;
 namespace HyperFlow {
 
 typedef std::vector<double> Vec1D;
+typedef std::vector<Vec1D> Vec2D;
+typedef std::vector<Vec2D> Vec3D;   // fields[x][y][var] (src/share/tensor/tensor.h:114)
 
 class Device;
 
@@ -126,9 +134,14 @@ public:
     double get_y_top() const { return extents[3]; }
     Vec1D get_centroid(unsigned i, unsigned j) const;        // mesh_block.cpp:239-257
 
-    // Padded conserved fields, [var][y][x]; reads the device state when bound.
-    const std::vector<double>& get_fields() const;
-    void set_fields(const std::vector<double>& fields);
+    // MeshBlock::get_fields / set_fields (mesh_block.cpp:155-164): the padded
+    // conserved fields as fields[x][y][var]; reads / writes the device state
+    // when bound.
+    const Vec3D& get_fields() const;
+    void set_fields(const Vec3D& fields);
+    // The same fields as one flat array [var][y][x] (x fastest).
+    const std::vector<double>& get_field_array() const;
+    void set_field_array(const std::vector<double>& fields);
     // fields[x][y][var] of the reference
     double get_flow_value(unsigned i, unsigned j, unsigned var) const;
     // interior cells from an initial condition (mesh_block.cpp:215-224)
@@ -143,7 +156,9 @@ private:
     unsigned x_cells, y_cells, ghost_cells, dimension;
     double dx, dy;
     mutable std::vector<double> fields;
+    mutable Vec3D fields3;                // the Vec3D view, rebuilt when stale
     mutable bool host_current = true;
+    mutable bool fields3_current = false;
     Device* device = nullptr;
     int index = -1;
 };
@@ -193,7 +208,9 @@ class TimeStep;
 struct DeviceOptions {
     int precision = HF_FP64;
     int dt_mode = HF_DT_COMPAT;
-    int tile_rows = 0;
+    int tile_rows = 0;   // work plan of the update launches (hf_case_desc); 0 = library default
+    int tail_rows = 0;
+    int fix_rows = 0;
 };
 
 // mesh/mesh/mesh.h:11-44
@@ -269,8 +286,11 @@ private:
 class Scheme {
 public:
     virtual ~Scheme() {}
-    // spatial update L of a padded block, [var][y][x], zero in the ghosts
-    virtual Vec1D operator()(const std::shared_ptr<MeshBlock>& block) = 0;
+    // Scheme::operator() (scheme.h:20): the spatial update L of a padded
+    // block, [x][y][var], zero in the ghosts
+    virtual Vec3D operator()(const std::shared_ptr<MeshBlock>& block);
+    // the same as a flat [var][y][x] array
+    virtual Vec1D residual(const std::shared_ptr<MeshBlock>& block) = 0;
     virtual double calculate_time_step(const std::shared_ptr<Mesh>& mesh) = 0;
     virtual int scheme_kind() const = 0;                     // HF_GODUNOV | HF_MUSCL_HANCOCK
     virtual int limiter_kind() const { return HF_MINBEE; }
@@ -280,7 +300,7 @@ class GodunovScheme : public Scheme {                       // scheme/godunov/go
 public:
     GodunovScheme(const std::shared_ptr<Model>& model, const std::shared_ptr<RiemannSolver>& riemann,
                   const std::shared_ptr<TimeStep>& time_step);
-    Vec1D operator()(const std::shared_ptr<MeshBlock>& block) override;
+    Vec1D residual(const std::shared_ptr<MeshBlock>& block) override;
     double calculate_time_step(const std::shared_ptr<Mesh>& mesh) override;
     int scheme_kind() const override { return HF_GODUNOV; }
     static const unsigned ghost_cells = 1;                   // godunov.cpp:42
@@ -296,7 +316,7 @@ public:
     MUSCLHancockScheme(const std::shared_ptr<Model>& model, const std::shared_ptr<RiemannSolver>& riemann,
                        const std::shared_ptr<TimeStep>& time_step, const std::shared_ptr<Limiter>& limiter,
                        const std::shared_ptr<Mesh>& mesh);
-    Vec1D operator()(const std::shared_ptr<MeshBlock>& block) override;
+    Vec1D residual(const std::shared_ptr<MeshBlock>& block) override;
     double calculate_time_step(const std::shared_ptr<Mesh>& mesh) override;
     int scheme_kind() const override { return HF_MUSCL_HANCOCK; }
     int limiter_kind() const override { return limiter->kind(); }
@@ -310,16 +330,22 @@ private:
     std::shared_ptr<Mesh> mesh;
 };
 
-// scheme/ode/*: ForwardEuler over every block of the mesh, blocks in order.
+// scheme/ode/*
 class ODESolver {
 public:
     virtual ~ODESolver() {}
-    // the per-block loop of Simulation::solve (simulation.cpp:106-110)
+    // ODESolver::operator() (ode.h:22-24): one step of a block's fields from
+    // the spatial update func() returns
+    virtual Vec3D operator()(const std::function<Vec3D()>& func, const Vec3D& val_to_step, double dt) = 0;
+    // the per-block loop of Simulation::solve (simulation.cpp:106-110) on the
+    // device, every block of the mesh in order
     virtual void advance(const std::shared_ptr<Mesh>& mesh, double dt) = 0;
 };
 
 class ForwardEulerODESolver : public ODESolver {            // forward_euler.cpp:15-33
 public:
+    // (dt * L) + U element by element (forward_euler.cpp:27-28)
+    Vec3D operator()(const std::function<Vec3D()>& func, const Vec3D& val_to_step, double dt) override;
     void advance(const std::shared_ptr<Mesh>& mesh, double dt) override;
 };
 
@@ -378,7 +404,8 @@ private:
 // ------------------------------------------------------------------ parser --
 // simulation/parser/parser.h: the string-keyed factories of the JSON case.
 // Build-only options live under an optional "hyperflow_amd" object:
-// {"precision": "fp64"|"fp32", "dt_mode": "compat"|"uniform", "tile_rows": n}.
+// {"precision": "fp64"|"fp32", "dt_mode": "compat"|"uniform", "tile_rows": n, "tail_rows": n,
+//  "fix_rows": n}.
 class JSONParser {
 public:
     explicit JSONParser(const hf_json::Value& config);

@@ -77,7 +77,7 @@ def test_net_desc_layout_and_param_count():
 
 def test_struct_layout_matches_header():
     assert C.sizeof(native.hf_block_desc) == 4 * 8 + 2 * 4 + 4 * 4 + 4 * 4 + 16 * 8 + 2 * 4
-    assert C.sizeof(native.hf_case_desc) == 4 * 4 + 2 * 8 + 2 * 4 + 8
+    assert C.sizeof(native.hf_case_desc) == 4 * 4 + 2 * 8 + 4 * 4 + 8
 
 
 def test_version_string():
@@ -89,7 +89,7 @@ def _desc(**over):
     arr = native.block_descs([case])
     d = native.hf_case_desc()
     d.scheme, d.limiter, d.precision, d.dt_mode = 0, 0, 0, 0
-    d.gamma, d.cfl, d.nblocks, d.tile_rows = 1.4, 0.5, 1, 0
+    d.gamma, d.cfl, d.nblocks, d.tile_rows, d.tail_rows, d.fix_rows = 1.4, 0.5, 1, 0, 0, 0
     d.blocks = C.cast(arr, C.POINTER(native.hf_block_desc))
     for k, v in over.items():
         if k.startswith("block_"):
@@ -108,6 +108,8 @@ def _desc(**over):
     ({"block_nx": 0}, "nx"),
     ({"block_x_right": -1.0}, "extents"),
     ({"block_problem": 1}, "problems"),
+    ({"tail_rows": -1}, "tail_rows"),
+    ({"fix_rows": -3}, "fix_rows"),
 ])
 def test_invalid_descriptors_are_rejected(over, msg):
     d, arr = _desc(**over)

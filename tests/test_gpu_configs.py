@@ -23,8 +23,8 @@ pytestmark = pytest.mark.gpu
 
 
 def test_config3_full_size_first_steps_against_oracle():
-    case = C.muscl_quadrants(0, 8192)
-    f = C.perturbed_fields(case, 0)
+    case = C.muscl_quadrants(accuracy.C3_SEED, 8192)
+    f = C.perturbed_fields(case, accuracy.C3_SEED)
     say("config3 full size 8192^2: 2 steps fp64 vs oracle")
     s, out, hist = gpu_run(case, f, 2)
     s.close()

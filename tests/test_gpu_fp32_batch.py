@@ -131,10 +131,16 @@ def test_graph_replays_match_eager_fp32():
 
 
 def test_autotuner_accepts_only_bitwise_identical_candidates(tmp_path, monkeypatch):
+    """Every candidate plan reproduces the default plan's fields (none is
+    rejected), the chosen plan is stored under the workload's class with its
+    timing spread, and a workload of the same class is served from it."""
     monkeypatch.setattr(autotune, "CACHE", str(tmp_path / "tuning.json"))
     case = C.step_channel(48, "MinBee", 0.5)
     say("autotune")
-    res = autotune.tune(case, case.initial_fields(), candidates=(16, 8, 32), steps=5, warmup=1)
+    res = autotune.tune(case, case.initial_fields(), rows=(16, 8, 32), tails=(4, 8), fixes=(0, 6), steps=5,
+                        warmup=1, repeats=2)
     assert res["rejected"] == []
-    assert res["tile_rows"] in (8, 16, 32)
-    assert autotune.cached_rows(case, "fp64") == res["tile_rows"]
+    assert res["tile_rows"] in (8, 16, 32) and res["tail_rows"] in (0, 4, 8) and res["fix_rows"] in (0, 6)
+    assert len(res["timings"]) >= 5 and 0.0 <= res["spread"] < 1.0
+    same_class = C.step_channel(50, "MinBee", 0.5)
+    assert autotune.cached_params(same_class, "fp64") == {p: res[p] for p in autotune.PARAMS}
