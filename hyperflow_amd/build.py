@@ -34,6 +34,7 @@ UNITS = [
     ("hf_kernels_f64.hip", DEVICE + ["-ffp-contract=off"], True),
     ("hf_kernels_f32.hip", DEVICE + ["-ffp-contract=off"], True),
     ("hf_cnn.hip", DEVICE + ["-ffp-contract=off"], True),
+    ("hf_noalloc.hip", DEVICE, True),
     ("hf_capi.cpp", ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-O2", "-ffp-contract=off"], True),
     ("hf_learned.cpp", ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-O2"], True),
     ("hf_uns.hip", DEVICE + ["-ffp-contract=off"], True),

@@ -28,6 +28,7 @@
 #include "hf_common.h"
 #include "hf_context.h"
 #include "hf_launch.h"
+#include "hf_noalloc.h"
 
 thread_local std::string g_err;
 
@@ -1061,6 +1062,115 @@ int hf_conservation(hf_ctx* c, int p, double sums[4])
             }
     }
     for (int k = 0; k < 4; k++) sums[k] = S[k] + Cm[k];
+    return HF_OK;
+}
+
+int hf_step_noalloc(const hf_case_desc* desc, const void* d_in, void* d_out, int64_t* calls, double* d_dt,
+                    void* stream)
+{
+    if (!desc || !d_in || !d_out || !calls) return fail(HF_EINVAL, "hf_step_noalloc: null argument");
+    if (d_in == d_out) return fail(HF_EINVAL, "hf_step_noalloc: input and output must be different arrays");
+    int rc = validate(desc);
+    if (rc) return rc;
+    // the host-side plan only: this context owns no device memory
+    hf_ctx c;
+    rc = build_plan(&c, desc, true);
+    if (rc) return rc;
+    std::vector<HfGhost> fill, copy;
+    build_tables(c, fill, copy);
+    for (int p = 0; p < c.np; p++)
+        if (calls[p] < 0 || calls[p] > (int64_t)UINT32_MAX - c.probs[p].cps)
+            return fail(HF_EINVAL, "hf_step_noalloc: TimeStep call counter out of range");
+
+    // Scratch layout: every region 256-byte aligned.  The zeroed arrays and the
+    // call counters are part of the host image, so one copy sets everything.
+    size_t used = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = (size_t)round_up((int64_t)used, 256);
+        used = o + bytes;
+        return o;
+    };
+    const size_t nb = (size_t)c.nb, np = (size_t)c.np;
+    const size_t o_blocks = take(sizeof(HfBlockDev) * nb), o_probs = take(sizeof(HfProbDev) * np),
+                 o_items = take(sizeof(HfItem) * c.items.size()), o_fill = take(sizeof(HfGhost) * fill.size()),
+                 o_consts = take(c.esize * 16 * nb), o_imax = take(sizeof(unsigned long long) * 4 * nb),
+                 o_gmax = take(sizeof(unsigned long long) * 4 * nb),
+                 o_fix = take(sizeof(unsigned long long) * 2 * nb), o_fixed = take(sizeof(uint32_t) * nb),
+                 o_skip = take(sizeof(uint32_t) * nb), o_skipn = take(sizeof(uint32_t) * nb),
+                 o_sync = take(sizeof(uint32_t) * 2 * HF_SYNC_WORDS),
+                 o_rounds = take(sizeof(int32_t) * (c.nrounds + 2)), o_pend = take(sizeof(uint32_t) * c.npending),
+                 o_calls = take(sizeof(uint32_t) * np);
+    if (used > HF_NOALLOC_SCRATCH_BYTES)
+        return fail(HF_EINVAL, "hf_step_noalloc: the case needs " + std::to_string(used) +
+                                   " bytes of device scratch, more than the maximum of " +
+                                   std::to_string((size_t)HF_NOALLOC_SCRATCH_BYTES));
+    void* base = nullptr;
+    std::string err;
+    if (hf_noalloc_scratch(&base, &err)) return fail(HF_EHIP, "hf_step_noalloc: " + err);
+
+    std::vector<unsigned char> img(used, 0);
+    std::memcpy(img.data() + o_blocks, c.blocks.data(), sizeof(HfBlockDev) * nb);
+    std::memcpy(img.data() + o_probs, c.probs.data(), sizeof(HfProbDev) * np);
+    std::memcpy(img.data() + o_items, c.items.data(), sizeof(HfItem) * c.items.size());
+    std::memcpy(img.data() + o_fill, fill.data(), sizeof(HfGhost) * fill.size());
+    for (size_t b = 0; b < nb; b++)
+        for (int sd = 0; sd < 4; sd++)
+            for (int v = 0; v < 4; v++) {
+                const size_t k = (b * 4 + sd) * 4 + v;
+                if (c.prec == HF_FP64) {
+                    const double x = c.bdesc[b].state[sd][v];
+                    std::memcpy(img.data() + o_consts + 8 * k, &x, 8);
+                } else {
+                    const float x = (float)c.bdesc[b].state[sd][v];
+                    std::memcpy(img.data() + o_consts + 4 * k, &x, 4);
+                }
+            }
+    for (size_t p = 0; p < np; p++) {
+        const uint32_t k = (uint32_t)calls[p];
+        std::memcpy(img.data() + o_calls + 4 * p, &k, 4);
+    }
+    {
+        const std::vector<int32_t> rt = round_table(c);
+        std::memcpy(img.data() + o_rounds, rt.data(), sizeof(int32_t) * rt.size());
+    }
+
+    unsigned char* S = (unsigned char*)base;
+    c.stream = (hipStream_t)stream;
+    c.buf[0] = const_cast<void*>(d_in);   // read only: the ghost fill and the update write buf[1]
+    c.buf[1] = d_out;
+    c.cur = 0;
+    c.d_blocks = (HfBlockDev*)(S + o_blocks);
+    c.d_probs = (HfProbDev*)(S + o_probs);
+    c.d_items = (HfItem*)(S + o_items);
+    c.d_fill = (HfGhost*)(S + o_fill);
+    c.nfill = (int)fill.size();
+    c.d_consts = S + o_consts;
+    c.d_imax = (unsigned long long*)(S + o_imax);
+    c.d_gmax = (unsigned long long*)(S + o_gmax);
+    c.d_imax_fix = (unsigned long long*)(S + o_fix);
+    c.d_fixed = (uint32_t*)(S + o_fixed);
+    c.d_skip = (uint32_t*)(S + o_skip);
+    c.d_skip_next = (uint32_t*)(S + o_skipn);
+    c.d_sync = (uint32_t*)(S + o_sync);
+    c.d_rounds = (int32_t*)(S + o_rounds);
+    c.d_pending = (uint32_t*)(S + o_pend);
+    c.d_calls = (uint32_t*)(S + o_calls);
+    c.d_hist = d_dt;
+    c.hist_cap = d_dt ? c.probs[0].cps : 0;
+    c.hist_base = calls[0];
+
+    HF_HIP(hipMemcpyAsync(base, img.data(), used, hipMemcpyHostToDevice, c.stream));
+    // wave-speed maxima of the input's interior (a device-resident loop gets
+    // them from the previous step's update epilogue)
+    for (int b = 0; b < c.nb; b++) {
+        rc = launch_scan(&c, b, d_in, d_in, 1, c.imax(0) + 2 * b);
+        if (rc) return rc;
+    }
+    rc = enqueue_steps(&c, 1, false);
+    if (rc) return rc;
+    // the host image must outlive its copy, and the caller gets a finished step
+    HF_HIP(hipStreamSynchronize(c.stream));
+    for (size_t p = 0; p < np; p++) calls[p] += c.probs[p].cps;
     return HF_OK;
 }
 

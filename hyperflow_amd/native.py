@@ -28,7 +28,7 @@ HF_RUN_GRAPH = 1
 
 EXPORTS = ("hf_create", "hf_destroy", "hf_ghost_cells", "hf_upload", "hf_download", "hf_apply_bcs",
            "hf_time_step", "hf_scheme_residual", "hf_advance", "hf_run", "hf_synchronize", "hf_dt_history",
-           "hf_calls", "hf_profile", "hf_kernel_stats", "hf_conservation", "hf_last_error",
+           "hf_calls", "hf_profile", "hf_kernel_stats", "hf_conservation", "hf_step_noalloc", "hf_last_error",
            "hf_version")
 # include/hyperflow_learned.h
 EXPORTS_LEARNED = ("hf_net_create", "hf_net_destroy", "hf_net_set_stream", "hf_net_set_desc", "hf_net_param_count",
@@ -87,6 +87,7 @@ def lib():
         L.hf_profile.argtypes = [vp, C.c_int]
         L.hf_kernel_stats.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
         L.hf_conservation.argtypes = [vp, C.c_int, _dp]
+        L.hf_step_noalloc.argtypes = [C.POINTER(hf_case_desc), vp, vp, C.POINTER(C.c_int64), vp, vp]
         L.hf_last_error.restype = C.c_char_p
         L.hf_last_error.argtypes = []
         L.hf_version.restype = C.c_char_p
@@ -168,6 +169,25 @@ def case_desc(cases, precision: str = "fp64", dt_mode: str = "compat", tile_rows
     d.fix_rows = fix_rows
     d.blocks = C.cast(descs, C.POINTER(hf_block_desc))
     return d, descs
+
+
+def step_noalloc(cases, d_in, d_out, calls, d_dt=None, precision: str = "fp64", dt_mode: str = "compat",
+                 stream: int = 0, tile_rows: int = 0) -> None:
+    """One step through hf_step_noalloc: d_in / d_out are device buffers (a
+    tensor or an address) of the blocks' padded fields back to back; calls is
+    an int64 numpy array of per-problem TimeStep counters, advanced in place."""
+    d, keep = case_desc(cases, precision, dt_mode, tile_rows)
+    if not (isinstance(calls, np.ndarray) and calls.dtype == np.int64 and calls.flags.c_contiguous):
+        raise ValueError("calls must be a contiguous int64 numpy array")
+    rc = lib().hf_step_noalloc(C.byref(d), C.c_void_p(_addr(d_in)), C.c_void_p(_addr(d_out)),
+                               calls.ctypes.data_as(C.POINTER(C.c_int64)),
+                               C.c_void_p(_addr(d_dt) if d_dt is not None else None), C.c_void_p(stream or None))
+    del keep
+    _check(rc, "hf_step_noalloc")
+
+
+def _addr(t) -> int:
+    return t.data_ptr() if hasattr(t, "data_ptr") else int(t)
 
 
 class Solver:

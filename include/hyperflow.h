@@ -147,6 +147,28 @@ HF_API int hf_kernel_stats(hf_ctx* ctx, int64_t* launches, double* total_ms);
  * cells of one problem, reduced on the device in a fixed order. */
 HF_API int hf_conservation(hf_ctx* ctx, int problem, double sums[4]);
 
+/* One step of hf_run (apply_bcs, TimeStep, the per-block ForwardEuler(Scheme)
+ * loop) without a context and without allocating device memory.  No
+ * counterpart in the reference: this is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).
+ *  - d_in / d_out: device arrays of the blocks back to back, each block
+ *    [4][ny+2g][nx+2g] (x fastest), double for HF_FP64 and float for HF_FP32.
+ *    d_in is only read; d_out receives the state hf_run leaves after one
+ *    step (new interior, ghosts filled from d_in).  Alternate them to step on.
+ *  - calls: host array, one TimeStep call counter per problem (the calls
+ *    made so far; 0 for a fresh start), advanced by this step's calls.
+ *  - d_dt: NULL or a device array receiving problem 0's TimeStep results of
+ *    this step in call order (1 for Godunov, 1 + blocks for MUSCL-Hancock).
+ *  - stream: a hipStream_t, or NULL for the null stream.  Returns when the
+ *    step is complete.
+ * Tables and working arrays live in one static device array per GPU (32 MiB);
+ * a case that needs more is rejected with HF_EINVAL.  Every call on a GPU
+ * shares that array: calls that overlap on one GPU (other threads or
+ * streams) race and must be serialised by the caller. */
+HF_API int hf_step_noalloc(const hf_case_desc* desc, const void* d_in, void* d_out, int64_t* calls, double* d_dt,
+                           void* stream);
+
 /* Message of the last failed call on this thread ("" if none). */
 HF_API const char* hf_last_error(void);
 /* Library/device description, e.g. "hyperflow_amd gfx950". */
