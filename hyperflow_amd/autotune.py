@@ -199,30 +199,34 @@ def tune(cases, fields: Sequence[np.ndarray], precision: str = "fp64", dt_mode: 
 
 def _workloads(names):
     """The benchmark workloads by name: (label, cases, fields, precision,
-    dt_mode, graph), graph as bench.py launches the workload."""
+    dt_mode, graph, search), graph as bench.py launches the workload, search
+    the candidate lists and repeats (fewer for the 8192^2 block, whose every
+    run moves 4 GB between host and device)."""
     from . import case as C
     out = []
     for name in names:
         if name == "godunov-quadrants":
             c = C.godunov_quadrants(0)
-            out.append(("godunov-quadrants", [c], [c.initial_fields()[0]], "fp64", "compat", False))
+            out.append(("godunov-quadrants", [c], [c.initial_fields()[0]], "fp64", "compat", False, {}))
         elif name == "step-channel":
             c = C.muscl_step_channel(1024)
-            out.append(("step-channel", [c], c.initial_fields(), "fp64", "compat", False))
+            out.append(("step-channel", [c], c.initial_fields(), "fp64", "compat", False, {}))
         elif name in ("muscl-quadrants", "muscl-quadrants-fp32"):
             from .accuracy import C3_SEED
             c = C.muscl_quadrants(C3_SEED, 8192)
             f = C.perturbed_fields(c, C3_SEED)
             for prec in (("fp64", "fp32") if name == "muscl-quadrants" else ("fp32",)):
-                out.append((f"muscl-quadrants-{prec}", [c], f, prec, "compat", False))
+                out.append((f"muscl-quadrants-{prec}", [c], f, prec, "compat", False,
+                            {"rows": (32, 48, 64, 96), "tails": (4, 8, 16), "repeats": 2}))
         elif name == "batch":
             cases = [C.quad_case(nx, ny, 1000 + k, scheme="MUSCLHancock", limiter="VanLeer", cfl=0.5)
                      for k, (nx, ny) in enumerate(C.batched_sizes(512, 0))]
-            out.append(("batch", cases, [c.initial_fields()[0] for c in cases], "fp32", "compat", True))
+            out.append(("batch", cases, [c.initial_fields()[0] for c in cases], "fp32", "compat", True, {}))
         elif name == "learned":
             for n in (1024, 1536, 2048):
                 c = C.muscl_quadrants(0, n)
-                out.append((f"learned-{n}", [c], C.perturbed_fields(c, 0), "fp32", "uniform", True))
+                out.append((f"learned-{n}", [c], C.perturbed_fields(c, 0), "fp32", "uniform", True,
+                            {"repeats": 2}))
         else:
             raise SystemExit(f"unknown workload {name}")
     return out
@@ -237,10 +241,12 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=20, help="timed steps per run")
     ap.add_argument("--repeats", type=int, default=3, help="timed runs per candidate")
     args = ap.parse_args(argv)
-    for label, cases, fields, prec, mode, graph in _workloads(args.workloads):
+    for label, cases, fields, prec, mode, graph, search in _workloads(args.workloads):
         print(f"{label}: class {class_key(problem_class(cases, prec, mode))}", flush=True)
-        r = tune(cases, fields, precision=prec, dt_mode=mode, steps=args.steps, repeats=args.repeats, graph=graph,
-                 label=label, verbose=True)
+        search = dict(search)
+        repeats = search.pop("repeats", args.repeats)
+        r = tune(cases, fields, precision=prec, dt_mode=mode, steps=args.steps, repeats=repeats, graph=graph,
+                 label=label, verbose=True, **search)
         print(f"{label}: tile_rows {r['tile_rows']} tail_rows {r['tail_rows']} fix_rows {r['fix_rows']} "
               f"({r['ms_per_step']:.4f} ms/step, spread {100 * r['spread']:.1f} %, rejected {r['rejected']})",
               flush=True)

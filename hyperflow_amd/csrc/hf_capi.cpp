@@ -250,6 +250,10 @@ void build_tables(const hf_ctx& c, std::vector<HfGhost>& fill, std::vector<HfGho
 int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
 // ---------------------------------------------------------------- launches --
+// Offsets of the main-list segments and the fix-up ranges in plan_table().
+size_t segs_offset(const hf_ctx& c) { return (size_t)c.nrounds + 2; }
+size_t fixrange_offset(const hf_ctx& c) { return segs_offset(c) + c.segs.size(); }
+
 template <class R>
 HfStepArgs<R> step_args(hf_ctx* c, int item0, int nitems, const void* A, const void* G, void* O)
 {
@@ -264,9 +268,13 @@ HfStepArgs<R> step_args(hf_ctx* c, int item0, int nitems, const void* A, const v
     a.G = (const R*)G;
     a.O = (R*)O;
     a.nprob = c->np;
+    a.nblocks = c->nb;
     a.nmain = nitems;
     a.sync = c->d_sync;
     a.rounds = c->d_rounds;
+    a.segs = c->d_rounds + segs_offset(*c);
+    a.fixrange = c->d_rounds + fixrange_offset(*c);
+    a.nskip = nullptr;
     a.nrounds = 0;
     a.pending = c->d_pending;
     a.imax_fix = c->d_imax_fix;
@@ -286,51 +294,53 @@ HfStepArgs<R> step_args(hf_ctx* c, int item0, int nitems, const void* A, const v
     return a;
 }
 
-template <class R>
-int launch_update_t(hf_ctx* c, HfStepArgs<R>& a, bool resid, bool timed)
+// A pair of timing events from the context's pool; the pending list takes
+// them once the launches that stamp them are queued.
+int take_events(hf_ctx* c, hipEvent_t& e0, hipEvent_t& e1)
 {
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    timed = timed && a.nitems > 0;   // an empty launch is not issued
-    if (timed) {
-        if (c->ev_free.empty()) {
-            HF_HIP(hipEventCreate(&e0));
-            HF_HIP(hipEventCreate(&e1));
-        } else {
-            e0 = c->ev_free.back().first;
-            e1 = c->ev_free.back().second;
-            c->ev_free.pop_back();
-        }
+    if (c->ev_free.empty()) {
+        HF_HIP(hipEventCreate(&e0));
+        HF_HIP(hipEventCreate(&e1));
+    } else {
+        e0 = c->ev_free.back().first;
+        e1 = c->ev_free.back().second;
+        c->ev_free.pop_back();
     }
-    const int grid = std::max(1, std::min(c->wg_cap, (a.nitems + 3) / 4));
-    HF_HIP(hf::launch_update<R>(c->scheme, c->limiter, resid, a, grid, c->stream, e0, e1));
-    if (timed) c->ev_pending.emplace_back(e0, e1);
     return HF_OK;
 }
 
-// One persistent update launch over items [item0, item0+nitems), taking
-// tickets with the sync counters of slot `slot` (zeroed before the launch
-// runs).  With rounds, the items after the first nmain are the fix-up rounds.
-int launch_update(hf_ctx* c, int item0, int nitems, int nmain, bool rounds, const void* A, const void* G, void* O,
-                  bool resid, bool fixed, double dt_step, double dt_pred, bool timed, int slot)
+// One persistent update launch of kind `kind` (main list, guess or check)
+// over items [item0, item0+nitems), taking tickets with the sync counters of
+// slot `slot` (zeroed before the launch runs); the items after the first
+// nmain are the fix-up rounds.  e0 / e1 (optional) are stamped at the
+// dispatch's start / end.
+template <class R>
+int launch_update_r(hf_ctx* c, int kind, int item0, int nitems, int nmain, const void* A, const void* G, void* O,
+                    bool resid, bool fixed, double dt_step, double dt_pred, int slot, hipEvent_t e0, hipEvent_t e1)
 {
-    if (c->prec == HF_FP64) {
-        HfStepArgs<double> a = step_args<double>(c, item0, nitems, A, G, O);
-        a.use_fixed = fixed;
-        a.fixed_step = dt_step;
-        a.fixed_pred = dt_pred;
-        a.sync = c->d_sync + slot * HF_SYNC_WORDS;
-        a.nmain = nmain;
-        a.nrounds = rounds ? c->nrounds : 0;
-        return launch_update_t<double>(c, a, resid, timed);
-    }
-    HfStepArgs<float> a = step_args<float>(c, item0, nitems, A, G, O);
+    HfStepArgs<R> a = step_args<R>(c, item0, nitems, A, G, O);
     a.use_fixed = fixed;
     a.fixed_step = dt_step;
     a.fixed_pred = dt_pred;
     a.sync = c->d_sync + slot * HF_SYNC_WORDS;
     a.nmain = nmain;
-    a.nrounds = rounds ? c->nrounds : 0;
-    return launch_update_t<float>(c, a, resid, timed);
+    a.nrounds = kind == hf::HF_LAUNCH_MAIN ? 0 : c->nrounds;
+    // the step's main launch skips the blocks the ghost fill counted
+    if (slot == HF_SLOT_MAIN && !fixed) a.nskip = c->d_sync + HF_SLOT_MAIN * HF_SYNC_WORDS + HF_SYNC_NSKIP;
+    const int grid = std::max(1, std::min(c->wg_cap, (nitems + 3) / 4));
+    HF_HIP(hf::launch_update<R>(c->scheme, c->limiter, resid, kind, a, grid, c->stream, e0, e1));
+    return HF_OK;
+}
+
+int launch_update(hf_ctx* c, int kind, int item0, int nitems, int nmain, const void* A, const void* G, void* O,
+                  bool resid, bool fixed, double dt_step, double dt_pred, int slot, hipEvent_t e0 = nullptr,
+                  hipEvent_t e1 = nullptr)
+{
+    if (c->prec == HF_FP64)
+        return launch_update_r<double>(c, kind, item0, nitems, nmain, A, G, O, resid, fixed, dt_step, dt_pred, slot,
+                                       e0, e1);
+    return launch_update_r<float>(c, kind, item0, nitems, nmain, A, G, O, resid, fixed, dt_step, dt_pred, slot, e0,
+                                  e1);
 }
 
 // A launch outside the device-resident loop (class-level API): its sync
@@ -338,8 +348,16 @@ int launch_update(hf_ctx* c, int item0, int nitems, int nmain, bool rounds, cons
 int launch_update_host(hf_ctx* c, int item0, int nitems, const void* A, const void* G, void* O, bool resid,
                        double dt_step, double dt_pred, bool timed)
 {
-    HF_HIP(hipMemsetAsync(c->d_sync + HF_SYNC_WORDS, 0, sizeof(uint32_t) * HF_SYNC_WORDS, c->stream));
-    return launch_update(c, item0, nitems, nitems, false, A, G, O, resid, true, dt_step, dt_pred, timed, 1);
+    HF_HIP(hipMemsetAsync(c->d_sync + HF_SLOT_HOST * HF_SYNC_WORDS, 0, sizeof(uint32_t) * HF_SYNC_WORDS, c->stream));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (timed && nitems > 0) {
+        const int rc = take_events(c, e0, e1);
+        if (rc) return rc;
+    }
+    const int rc = launch_update(c, hf::HF_LAUNCH_MAIN, item0, nitems, nitems, A, G, O, resid, true, dt_step, dt_pred,
+                                 HF_SLOT_HOST, e0, e1);
+    if (e0) c->ev_pending.emplace_back(e0, e1);
+    return rc;
 }
 
 int launch_ghost(hf_ctx* c, const HfGhost* tab, int n, const void* S, void* D, unsigned long long* gmax,
@@ -413,8 +431,8 @@ int ensure_partials(hf_ctx* c)
     if (c->partials_valid) return HF_OK;
     HF_HIP(hipMemsetAsync(c->imax(c->cur), 0, sizeof(unsigned long long) * 2 * c->nb, c->stream));
     HF_HIP(hipMemsetAsync(c->gmax(c->cur), 0, sizeof(unsigned long long) * 2 * c->nb, c->stream));
-    HF_HIP(hipMemsetAsync(c->d_imax_fix, 0, sizeof(unsigned long long) * 2 * c->nb, c->stream));
-    HF_HIP(hipMemsetAsync(c->d_fixed, 0, sizeof(uint32_t) * c->nb, c->stream));
+    HF_HIP(hipMemsetAsync(c->d_imax_fix, 0, sizeof(unsigned long long) * 4 * c->nb, c->stream));
+    HF_HIP(hipMemsetAsync(c->d_fixed, 0, sizeof(uint32_t) * 2 * c->nb, c->stream));
     HF_HIP(hipMemsetAsync(c->d_skip_next, 0, sizeof(uint32_t) * c->nb, c->stream));
     for (int b = 0; b < c->nb; b++) {
         int rc = launch_scan(c, b, c->buf[c->cur], c->buf[c->cur], 1, c->imax(c->cur) + 2 * b);
@@ -445,7 +463,7 @@ int enqueue_steps(hf_ctx* c, int nsteps, bool timed, bool fixed = false, double 
         bk.clear1 = c->imax(k ^ 1);
         bk.nclear = 2 * c->nb;
         bk.queue = c->d_sync;
-        bk.nqueue = HF_SYNC_WORDS;
+        bk.nqueue = HF_SLOTS * HF_SYNC_WORDS;
         bk.pending = c->d_pending;
         bk.npending = c->npending;
         bk.nblocks = c->nb;
@@ -454,12 +472,31 @@ int enqueue_steps(hf_ctx* c, int nsteps, bool timed, bool fixed = false, double 
         bk.fixed = c->d_fixed;
         bk.skip = c->d_skip;
         bk.skip_next = c->d_skip_next;
+        bk.fixstats = c->d_fixstats;
         int rc = launch_ghost(c, c->d_fill, c->nfill, A, B, c->gmax(k), bk);
         if (rc) return rc;
-        // the main list and, with the CFL dt, the fix-up rounds in one launch
-        const int n = fixed ? c->main_items : c->main_items + c->fix_items;
-        rc = launch_update(c, c->main_begin, n, c->main_items, !fixed, A, B, B, false, fixed, dt, dt, timed, 0);
+        // the main list, then with the CFL dt of a problem of several MUSCL
+        // blocks the guess and check launches; timed from the first
+        // launch's start to the last one's end
+        const bool rounds = !fixed && c->nrounds > 0;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (timed) {
+            rc = take_events(c, e0, e1);
+            if (rc) return rc;
+        }
+        const int n = c->main_items + c->fix_items;
+        rc = launch_update(c, hf::HF_LAUNCH_MAIN, c->main_begin, c->main_items, c->main_items, A, B, B, false, fixed,
+                           dt, dt, HF_SLOT_MAIN, e0, rounds ? nullptr : e1);
         if (rc) return rc;
+        if (rounds) {
+            rc = launch_update(c, hf::HF_LAUNCH_GUESS, c->main_begin, n, c->main_items, A, B, B, false, false, 0.0,
+                               0.0, HF_SLOT_GUESS);
+            if (rc) return rc;
+            rc = launch_update(c, hf::HF_LAUNCH_CHECK, c->main_begin, n, c->main_items, A, B, B, false, false, 0.0,
+                               0.0, HF_SLOT_CHECK, nullptr, e1);
+            if (rc) return rc;
+        }
+        if (timed) c->ev_pending.emplace_back(e0, e1);
         c->cur ^= 1;
     }
     c->cur = c0;
@@ -512,13 +549,18 @@ void destroy_graphs(hf_ctx* c)
     c->graphs.clear();
 }
 
-// First item of each fix-up round relative to the main list's first item,
-// [1..nrounds], and the end of the last round at [nrounds + 1].
-std::vector<int32_t> round_table(const hf_ctx& c)
+// The device plan table of the update launches, item positions relative to
+// the main list's first item: the first item of each fix-up round at
+// [1..nrounds] and the end of the last round at [nrounds + 1]; then from
+// segs_offset() the main list's (first, count) per phase and block, and from
+// fixrange_offset() each block's (first, count) in the fix-up rounds.
+std::vector<int32_t> plan_table(const hf_ctx& c)
 {
     std::vector<int32_t> rt(c.nrounds + 2, 0);
     for (int l = 1; l <= c.nrounds; l++) rt[l] = c.fix_rounds[l - 1].first - c.main_begin;
     rt[c.nrounds + 1] = c.main_items + c.fix_items;
+    rt.insert(rt.end(), c.segs.begin(), c.segs.end());
+    rt.insert(rt.end(), c.fixrange.begin(), c.fixrange.end());
     return rt;
 }
 
@@ -649,12 +691,20 @@ int build_plan(hf_ctx* c, const hf_case_desc* desc, bool dense)
             y1[b] = std::min(ny, (int)(ny * (1.0 - f2 - f3)) / rows * rows);
             y2[b] = std::min(ny, y1[b] + std::max(0, (int)(ny * (1.0 - f3)) - y1[b]) / r2 * r2);
         }
-        for (int b = 0; b < c->nb; b++) add_rows(b, 0, y1[b], rows, true);
-        for (int b = 0; b < c->nb; b++) add_rows(b, y1[b], y2[b], r2, true);
-        for (int b = 0; b < c->nb; b++) add_rows(b, y2[b], c->blocks[b].ny, r3, true);
+        c->segs.assign(6 * (size_t)c->nb, 0);
+        for (int ph = 0; ph < 3; ph++)
+            for (int b = 0; b < c->nb; b++) {
+                const int first = (int)c->items.size();
+                if (ph == 0) add_rows(b, 0, y1[b], rows, true);
+                else if (ph == 1) add_rows(b, y1[b], y2[b], r2, true);
+                else add_rows(b, y2[b], c->blocks[b].ny, r3, true);
+                c->segs[2 * (ph * c->nb + b)] = first - perblock_end;
+                c->segs[2 * (ph * c->nb + b) + 1] = (int)c->items.size() - first;
+            }
     }
     c->main_begin = perblock_end;
     c->main_items = (int)c->items.size() - perblock_end;
+    c->fixrange.assign(2 * (size_t)c->nb, 0);
     if (c->scheme == HF_MUSCL_HANCOCK && c->dt_mode == HF_DT_COMPAT)
         for (int lb = 1; lb < c->maxnb; lb++) {
             // rows per item so that the launch's wave-rows fill about 95 % of
@@ -670,7 +720,12 @@ int build_plan(hf_ctx* c, const hf_case_desc* desc, bool dense)
                                : (int)std::max<int64_t>(4, std::min<int64_t>(16, (work + target - 1) / target));
             const int first = (int)c->items.size();
             for (const auto& P : c->probs)
-                if (P.nb > lb) add_rows(P.first + lb, 0, c->blocks[P.first + lb].ny, std::min(rows, fr), true);
+                if (P.nb > lb) {
+                    const int b = P.first + lb, bfirst = (int)c->items.size();
+                    add_rows(b, 0, c->blocks[b].ny, std::min(rows, fr), true);
+                    c->fixrange[2 * b] = bfirst - c->main_begin;
+                    c->fixrange[2 * b + 1] = (int)c->items.size() - bfirst;
+                }
             c->fix_rounds.emplace_back(first, (int)c->items.size() - first);
         }
     // the fix-up rounds follow the main list in the item array
@@ -753,18 +808,20 @@ int hf_create(const hf_case_desc* desc, hf_ctx** out)
     HF_HIPC(hipMalloc(&C->d_scan, sizeof(unsigned long long) * 4 * C->nb));
     HF_HIPC(hipMemset(C->d_imax, 0, sizeof(unsigned long long) * 4 * C->nb));
     HF_HIPC(hipMemset(C->d_gmax, 0, sizeof(unsigned long long) * 4 * C->nb));
-    HF_HIPC(hipMalloc(&C->d_imax_fix, sizeof(unsigned long long) * 2 * C->nb));
-    HF_HIPC(hipMemset(C->d_imax_fix, 0, sizeof(unsigned long long) * 2 * C->nb));
-    HF_HIPC(hipMalloc(&C->d_fixed, sizeof(uint32_t) * C->nb));
-    HF_HIPC(hipMemset(C->d_fixed, 0, sizeof(uint32_t) * C->nb));
+    HF_HIPC(hipMalloc(&C->d_imax_fix, sizeof(unsigned long long) * 4 * C->nb));
+    HF_HIPC(hipMemset(C->d_imax_fix, 0, sizeof(unsigned long long) * 4 * C->nb));
+    HF_HIPC(hipMalloc(&C->d_fixed, sizeof(uint32_t) * 2 * C->nb));
+    HF_HIPC(hipMalloc(&C->d_fixstats, sizeof(unsigned long long) * 4));
+    HF_HIPC(hipMemset(C->d_fixstats, 0, sizeof(unsigned long long) * 4));
+    HF_HIPC(hipMemset(C->d_fixed, 0, sizeof(uint32_t) * 2 * C->nb));
     for (uint32_t** p : {&C->d_skip, &C->d_skip_next}) {
         HF_HIPC(hipMalloc(p, sizeof(uint32_t) * C->nb));
         HF_HIPC(hipMemset(*p, 0, sizeof(uint32_t) * C->nb));
     }
-    HF_HIPC(hipMalloc(&C->d_sync, sizeof(uint32_t) * 2 * HF_SYNC_WORDS));
-    HF_HIPC(hipMemset(C->d_sync, 0, sizeof(uint32_t) * 2 * HF_SYNC_WORDS));
+    HF_HIPC(hipMalloc(&C->d_sync, sizeof(uint32_t) * HF_SLOTS * HF_SYNC_WORDS));
+    HF_HIPC(hipMemset(C->d_sync, 0, sizeof(uint32_t) * HF_SLOTS * HF_SYNC_WORDS));
     {
-        const std::vector<int32_t> rt = round_table(*C);
+        const std::vector<int32_t> rt = plan_table(*C);
         HF_HIPC(hipMalloc(&C->d_rounds, sizeof(int32_t) * rt.size()));
         HF_HIPC(hipMemcpy(C->d_rounds, rt.data(), sizeof(int32_t) * rt.size(), hipMemcpyHostToDevice));
     }
@@ -799,7 +856,7 @@ void hf_destroy(hf_ctx* c)
     for (void* p : {c->buf[0], c->buf[1], (void*)c->d_blocks, (void*)c->d_probs, (void*)c->d_items, (void*)c->d_fill,
                     (void*)c->d_copy, c->d_consts, (void*)c->d_imax, (void*)c->d_gmax, (void*)c->d_scan,
                     (void*)c->d_calls, (void*)c->d_hist, (void*)c->d_part, c->d_resid, (void*)c->d_imax_fix,
-                    (void*)c->d_fixed, (void*)c->d_sync, (void*)c->d_rounds, (void*)c->d_pending, (void*)c->d_skip,
+                    (void*)c->d_fixed, (void*)c->d_fixstats, (void*)c->d_sync, (void*)c->d_rounds, (void*)c->d_pending, (void*)c->d_skip,
                     (void*)c->d_skip_next})
         if (p) (void)hipFree(p);
     if (c->h_calls_pinned) (void)hipHostFree(c->h_calls_pinned);
@@ -1035,6 +1092,19 @@ int hf_kernel_stats(hf_ctx* c, int64_t* launches, double* total_ms)
     return HF_OK;
 }
 
+int hf_fixup_stats(hf_ctx* c, int64_t out[4])
+{
+    if (!c || !out) return fail(HF_EINVAL, "hf_fixup_stats: null argument");
+    unsigned long long h[4] = {0, 0, 0, 0};
+    if (c->d_fixstats) {
+        HF_HIP(hipStreamSynchronize(c->stream));
+        HF_HIP(hipMemcpy(h, c->d_fixstats, sizeof h, hipMemcpyDeviceToHost));
+        HF_HIP(hipMemset(c->d_fixstats, 0, sizeof h));
+    }
+    for (int k = 0; k < 4; k++) out[k] = (int64_t)h[k];
+    return HF_OK;
+}
+
 int hf_conservation(hf_ctx* c, int p, double sums[4])
 {
     if (!c || !sums) return fail(HF_EINVAL, "hf_conservation: null argument");
@@ -1095,10 +1165,10 @@ int hf_step_noalloc(const hf_case_desc* desc, const void* d_in, void* d_out, int
                  o_items = take(sizeof(HfItem) * c.items.size()), o_fill = take(sizeof(HfGhost) * fill.size()),
                  o_consts = take(c.esize * 16 * nb), o_imax = take(sizeof(unsigned long long) * 4 * nb),
                  o_gmax = take(sizeof(unsigned long long) * 4 * nb),
-                 o_fix = take(sizeof(unsigned long long) * 2 * nb), o_fixed = take(sizeof(uint32_t) * nb),
+                 o_fix = take(sizeof(unsigned long long) * 4 * nb), o_fixed = take(sizeof(uint32_t) * 2 * nb),
                  o_skip = take(sizeof(uint32_t) * nb), o_skipn = take(sizeof(uint32_t) * nb),
-                 o_sync = take(sizeof(uint32_t) * 2 * HF_SYNC_WORDS),
-                 o_rounds = take(sizeof(int32_t) * (c.nrounds + 2)), o_pend = take(sizeof(uint32_t) * c.npending),
+                 o_sync = take(sizeof(uint32_t) * HF_SLOTS * HF_SYNC_WORDS),
+                 o_rounds = take(sizeof(int32_t) * plan_table(c).size()), o_pend = take(sizeof(uint32_t) * c.npending),
                  o_calls = take(sizeof(uint32_t) * np);
     if (used > HF_NOALLOC_SCRATCH_BYTES)
         return fail(HF_EINVAL, "hf_step_noalloc: the case needs " + std::to_string(used) +
@@ -1130,7 +1200,7 @@ int hf_step_noalloc(const hf_case_desc* desc, const void* d_in, void* d_out, int
         std::memcpy(img.data() + o_calls + 4 * p, &k, 4);
     }
     {
-        const std::vector<int32_t> rt = round_table(c);
+        const std::vector<int32_t> rt = plan_table(c);
         std::memcpy(img.data() + o_rounds, rt.data(), sizeof(int32_t) * rt.size());
     }
 

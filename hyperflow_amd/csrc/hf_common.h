@@ -47,7 +47,7 @@ struct HfStepArgs {
     const HfProbDev* probs;
     const HfItem* items;
     int32_t item0, nitems;
-    int32_t nprob, pad0;
+    int32_t nprob, nblocks;
     const R* A;   // interior source (U^n)
     const R* G;   // ghost source (filled ghosts)
     R* O;         // destination: U^{n+1} interior, or the residual L
@@ -66,17 +66,32 @@ struct HfStepArgs {
     uint32_t* sync;                      // HF_SYNC_WORDS counters of this launch (zero at launch)
     const int32_t* rounds;               // rounds[l]: first item of fix-up round l (1..nrounds); rounds[nrounds+1] = nitems
     int32_t nrounds, pad1;
-    uint32_t* pending;                   // bitmap over the fix-up items left to the launch's last wave
-    unsigned long long* imax_fix;        // [nblocks][2] maxima of blocks recomputed in a fix-up round
-    uint32_t* fixed;                     // [nblocks] block was recomputed in a fix-up round this step
-    uint32_t* skip;                      // [nblocks] the main list leaves the block to its fix-up round
-    uint32_t* skip_next;                 // [nblocks] the fix-up verdict for the next step
+    // Item ranges (first, count): segs[2 * (phase * nblocks + b)] of block b
+    // in each of the main list's three phases, fixrange[2 * b] of block b in
+    // the fix-up rounds; nskip: blocks the main list skips this step.
+    const int32_t* segs;
+    const int32_t* fixrange;
+    const uint32_t* nskip;
+    uint32_t* pending;                   // bitmap over the fix-up items left to the check launch's last wave
+    // Fix-up state of the step.  Slot 0 belongs to the guess launch, slot 1 to
+    // the check launch: imax_fix[s * 2 * nblocks + 2 * b + axis] holds the
+    // maxima of block b recomputed by launch s, fixed[s * nblocks + b] says
+    // that it was.
+    unsigned long long* imax_fix;
+    uint32_t* fixed;
+    uint32_t* skip;                      // [nblocks] the main list leaves the block to the guess launch
+    uint32_t* skip_next;                 // [nblocks] the exact verdict for the next step
 };
 
 // Counters of one update launch: the work-queue ticket counter, the number of
 // completed items (bit 31: the abort flag of the bounded waits) and the
 // number of waves that have left.
-enum { HF_SYNC_QUEUE = 0, HF_SYNC_DONE = 1, HF_SYNC_EXITED = 2, HF_SYNC_WORDS = 4 };
+// HF_SYNC_NSKIP (main slot only): blocks the step's main list skips, counted
+// by the ghost fill.
+enum { HF_SYNC_QUEUE = 0, HF_SYNC_DONE = 1, HF_SYNC_EXITED = 2, HF_SYNC_NSKIP = 3, HF_SYNC_WORDS = 4 };
+// Sync slots of a context: the step's main launch, class-level launches, the
+// step's guess launch and its check launch.
+enum { HF_SLOT_MAIN = 0, HF_SLOT_HOST = 1, HF_SLOT_GUESS = 2, HF_SLOT_CHECK = 3, HF_SLOTS = 4 };
 #define HF_SYNC_ABORT_BIT 0x80000000u
 
 // Step bookkeeping done by the first workgroup of a ghost-fill launch.
@@ -88,15 +103,20 @@ struct GhostBook {
     unsigned long long* clear0;   // arrays to zero (next step's partials)
     unsigned long long* clear1;
     int32_t nclear;               // elements in each
-    uint32_t* queue;              // sync counters of this step's update launch (HF_SYNC_WORDS)
+    uint32_t* queue;              // sync counters of this step's update launches (HF_SLOTS x HF_SYNC_WORDS)
     int32_t nqueue;
     uint32_t* pending;            // pending bitmap of this step's update launch
     int32_t npending;             // words
     int32_t nblocks;
-    // fold the previous step's fix-ups into its maxima, then clear them
+    // fold the previous step's fix-ups into its maxima (the check launch's
+    // over the guess launch's), then clear them
     unsigned long long* imax_prev;   // maxima of the state this step starts from
-    unsigned long long* imax_fix;
+    unsigned long long* imax_fix;    // both slots, as in HfStepArgs
     uint32_t* fixed;
     uint32_t* skip;               // skip <- skip_next for the step that starts
     uint32_t* skip_next;
+    // optional tallies of the previous step's fix-ups: steps, blocks the main
+    // list skipped, blocks the guess launch computed, blocks the check launch
+    // computed
+    unsigned long long* fixstats;
 };

@@ -38,6 +38,8 @@ struct hf_ctx {
     std::vector<HfItem> items;
     std::vector<int> item_begin, item_count;      // per block
     std::vector<std::pair<int, int>> fix_rounds;  // item ranges of the fix-up launches (lb = 1, 2, ..)
+    std::vector<int32_t> segs;       // [3][nb][2] (first, count) of each block in the main list's phases
+    std::vector<int32_t> fixrange;   // [nb][2] (first, count) of each block in the fix-up rounds
     int main_begin = 0, main_items = 0;           // item range of a step's main launch
     int maxnb = 1;
     int wg_cap = 1024;                            // resident workgroups of an update launch
@@ -57,14 +59,15 @@ struct hf_ctx {
     unsigned long long* d_imax = nullptr;   // [2][nb][2]
     unsigned long long* d_gmax = nullptr;   // [2][nb][2]
     unsigned long long* d_scan = nullptr;   // [nb][2] scan results + [nb][2] scratch maxima
-    unsigned long long* d_imax_fix = nullptr;   // [nb][2]
-    uint32_t* d_fixed = nullptr;                // [nb]
-    uint32_t* d_sync = nullptr;                 // [2][HF_SYNC_WORDS]: the step loop's launch, class-level launches
-    int32_t* d_rounds = nullptr;                // [nrounds + 2] first item of each fix-up round (main-list relative)
+    unsigned long long* d_imax_fix = nullptr;   // [2][nb][2] maxima of blocks recomputed by the guess / check launch
+    uint32_t* d_fixed = nullptr;                // [2][nb] block recomputed by the guess / check launch
+    uint32_t* d_sync = nullptr;                 // [HF_SLOTS][HF_SYNC_WORDS] launch counters (HF_SLOT_MAIN, ..)
+    int32_t* d_rounds = nullptr;                // plan_table(): round starts, main-list segments, fix-up ranges
     uint32_t* d_pending = nullptr;              // [npending] bitmap over the fix-up items
     int nrounds = 0, fix_items = 0, npending = 1;
-    uint32_t* d_skip = nullptr;                 // [nb] block left to its fix-up launch this step
+    uint32_t* d_skip = nullptr;                 // [nb] block left to the guess launch this step
     uint32_t* d_skip_next = nullptr;            // [nb]
+    unsigned long long* d_fixstats = nullptr;   // [4] fix-up tallies (GhostBook::fixstats)
     uint32_t* d_calls = nullptr;
     uint32_t* h_calls_pinned = nullptr;
     hipEvent_t calls_copied = nullptr;   // the last upload of h_calls_pinned has been read

@@ -64,22 +64,42 @@ __device__ __forceinline__ unsigned wave_id()
     // is compiled as divergent (vector loads of the item and block records)
     return __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
 }
-__device__ __forceinline__ unsigned next_ticket(uint32_t* sync, int lane, unsigned nstatic)
+__device__ __forceinline__ unsigned next_ticket(uint32_t* sync, int lane, unsigned base)
 {
     const unsigned t = atomicAdd(&sync[HF_SYNC_QUEUE], lane == 0 ? 1u : 0u);
-    return __builtin_amdgcn_readfirstlane(t) + nstatic;
+    return __builtin_amdgcn_readfirstlane(t) + base;
 }
-__device__ __forceinline__ unsigned first_ticket(uint32_t* sync, int lane, unsigned nstatic)
+// nstatic items are handed out without the counter; counter value c is then
+// item base + c (base = nstatic, or further on when the launch starts later
+// in the list)
+__device__ __forceinline__ unsigned first_ticket(uint32_t* sync, int lane, unsigned nstatic, unsigned base)
 {
     const unsigned w = wave_id();
-    return w < nstatic ? w : next_ticket(sync, lane, nstatic);
+    return w < nstatic ? w : next_ticket(sync, lane, base);
 }
 
-// One item of the launch is complete: its stores and maxima atomics are made
-// visible at device scope before the count that waiting waves poll.
+// What waves of one launch tell each other travels only through device-scope
+// atomics (wave-speed maxima, fixed flags, the pending bitmap, the counters),
+// which are coherent across the XCDs' L2 caches without cache maintenance:
+// writers update with atomic read-modify-writes, readers use atomic loads.
+// The new states themselves are not read inside the launch.
+__device__ __forceinline__ unsigned atomic_load(const uint32_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long atomic_load(const unsigned long long* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One item of the launch is complete.  This wave's atomics (its maxima and
+// fixed flag) must be performed before the count that waiting waves poll:
+// s_waitcnt vmcnt(0) waits until the memory system has acknowledged every
+// vector memory operation this wave issued (atomics without return
+// included), then the count is raised.
 __device__ __forceinline__ void item_done(uint32_t* sync, int lane, unsigned n = 1)
 {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __builtin_amdgcn_s_waitcnt(0xF70);   // vmcnt(0), expcnt and lgkmcnt unconstrained
     __hip_atomic_fetch_add(&sync[HF_SYNC_DONE], lane == 0 ? n : 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -95,13 +115,9 @@ __device__ __forceinline__ bool wait_done(uint32_t* sync, unsigned target)
 {
     const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
     for (;;) {
-        const unsigned d = __builtin_amdgcn_readfirstlane(
-            __hip_atomic_load(&sync[HF_SYNC_DONE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        const unsigned d = __builtin_amdgcn_readfirstlane(atomic_load(&sync[HF_SYNC_DONE]));
         if (d & HF_SYNC_ABORT_BIT) return false;
-        if (d >= target) {
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            return true;
-        }
+        if (d >= target) return true;
         if (__builtin_amdgcn_s_memrealtime() - t0 > WAIT_TICKS) {
             __hip_atomic_fetch_or(&sync[HF_SYNC_DONE], HF_SYNC_ABORT_BIT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return false;
@@ -347,41 +363,43 @@ __device__ __forceinline__ void muscl_item_t(const HfStepArgs<typename M::R>& a,
 #undef COL
 }
 
-// The fix-up verdict of the lb-th block of problem p: the step dt, the
-// speculative predictor dt the main list used (every block at U^n) and the
-// reference's (blocks before this one at their final U^{n+1}); the block must
-// be (re)computed in its fix-up round if the main list skipped it or the two
-// predictor dts differ.  Also records the verdict for the next step.
+// The guess launch's plan of problem p (plan_scan with the main list's
+// maxima): its first block to fix, the guess minimum m and the speculative
+// minimum O.
 template <class R>
-__device__ __forceinline__ bool fix_verdict(const HfStepArgs<R>& a, int p, const HfProbDev& pr, int lb, double& dt_step,
-                                            double& dt_pred, long long& kstep, long long& kpred)
+__device__ __forceinline__ int guess_plan(const HfStepArgs<R>& a, int p, const HfProbDev& pr, double& O, double& m,
+                                          long long& kstep)
 {
-    double spec;
-    step_dts_fix<R>(a, p, pr, lb, dt_step, spec, dt_pred, kstep, kpred);
-    const int b = pr.first + lb;
-    a.skip_next[b] = spec != dt_pred ? 1u : 0u;   // the same value from every wave
-    return a.skip[b] != 0u || spec != dt_pred;
+    kstep = (long long)a.calls[p] - pr.cps + 1;
+    double o = 1e16;
+    const int end = pr.first + pr.nb;
+    for (int c = pr.first + (int)(threadIdx.x & 63); c < end; c += 64) o = smin(o, term_old(a, c));
+    O = wave_min(o);
+    m = 0.0;
+    return plan_scan<R, PRE_MAIN>(a, pr, a.nblocks, kstep, O, m, false);
 }
 
-// Fix-up rounds from round l on, while they are empty: round l holds the
-// lb = l-th block of every problem with more than l blocks, and its verdicts
-// need the blocks before it final, which holds while every earlier round was
-// empty.  The queue counter jumps past each empty round (its unissued items
-// are counted complete).  Returns the first round with work, or nrounds + 1.
+// Check-launch rounds from round l on, while none of their blocks needs
+// computing: round l holds the l-th block of every problem with more than l
+// blocks, and its verdicts need the blocks before it final, which holds while
+// every earlier round has been completed.  The queue counter jumps past each
+// round passed over (its unissued items are counted complete).  Returns the
+// first round to compute, or nrounds + 1.
 template <class R>
-__device__ int first_busy_round(const HfStepArgs<R>& a, int l, int lane, unsigned nstatic)
+__device__ int first_busy_round(const HfStepArgs<R>& a, int l, int lane, unsigned base)
 {
     for (; l <= a.nrounds; l++) {
         bool need = false;
         for (int p = 0; p < a.nprob; p++) {
             const HfProbDev pr = a.probs[p];
             if (pr.nb <= l) continue;
-            double st, pd;
+            double O, mg, st, pd;
             long long ks, kp;
-            need = fix_verdict<R>(a, p, pr, l, st, pd, ks, kp) || need;
+            guess_plan<R>(a, p, pr, O, mg, ks);
+            need = check_verdict<R>(a, p, pr, a.nblocks, l, mg, st, pd, ks, kp) || need;
         }
         if (need) return l;
-        const unsigned end = (unsigned)a.rounds[l + 1] - nstatic;
+        const unsigned end = (unsigned)a.rounds[l + 1] - base;
         const unsigned old = __builtin_amdgcn_readfirstlane(atomicMax(&a.sync[HF_SYNC_QUEUE], end));
         if (old < end) item_done(a.sync, lane, end - old);
     }
@@ -397,19 +415,6 @@ __device__ __forceinline__ int round_of(const HfStepArgs<R>& a, unsigned idx)
     return l;
 }
 
-// Persistent MUSCL-Hancock launch: waves take work items from a queue until
-// it is empty.  The queue holds the main list (every block with the
-// speculative predictor dt, all blocks at U^n, except blocks marked skip[b]:
-// their speculation failed in the previous step), then the fix-up rounds of
-// the reference-compatible dt (muscl_hancock.cpp:246 with the sequential block
-// loop of simulation.cpp:106-110): round l recomputes the l-th block of each
-// problem with the reference's predictor dt, earlier blocks at their final
-// U^{n+1}, wherever the main list skipped it or its speculative dt differs.
-// An item of round l first waits (bounded) until every earlier item of the
-// queue is complete.  Empty rounds are skipped as a whole.  A wait that gives
-// up leaves its item to the launch's last wave, which computes every such
-// item in queue order once all other waves have left: no wait there, so the
-// launch always ends, and the results are the same bit for bit.
 template <class M, int LIM, bool RESID>
 __device__ __forceinline__ void muscl_go(const HfStepArgs<typename M::R>& a, const HfItem& it, int lane, double dt_step,
                                          double dt_pred, unsigned long long* imax_out)
@@ -431,91 +436,338 @@ __device__ __forceinline__ void record_dts(const HfStepArgs<R>& a, const HfItem&
     }
 }
 
+// MUSCL-Hancock step with the reference-compatible dt (muscl_hancock.cpp:246
+// with the sequential block loop of simulation.cpp:106-110: block l's
+// predictor dt sees blocks 0..l-1 of its problem at U^{n+1}) in three
+// persistent launches, each writing a block at most once (the XCDs' L2 caches
+// are written back in no fixed order, so two writers of one cache line in one
+// launch could leave either value):
+//  1. main (k_muscl): every block except those marked skip[b] (the previous
+//     step's verdict that the speculation fails), with the speculative
+//     predictor dt (every block at U^n).  No block waits for another.
+//  2. guess (k_muscl_guess): from the main list's maxima, each problem's
+//     first wrong block L and, for every block l >= L, the exact dt of block
+//     L scaled to l -- exact for l as long as blocks L..l-1 do not set a new
+//     minimum.  Every block whose dt differs from its speculative one (or
+//     that was skipped) is computed with it.  No waits.
+//  3. check (k_muscl_check): from the final maxima, the first block whose dt
+//     so far is not the exact one; normally none, and the launch returns at
+//     once.  Otherwise rounds follow from that block on: round l holds block
+//     l of every problem and waits (bounded) until every earlier round is
+//     complete, then recomputes the blocks whose dt is still wrong.
+// Items [0, nmain) form the main list, [nmain, nitems) the rounds: round l
+// lists the l-th block of every problem (the guess and check launches take
+// them in that order).  Without a CFL dt (use_fixed) or in uniform mode the
+// main launch is the whole step.
+// Exclusive prefix sum over the wave's lanes; total: the sum over all lanes.
+__device__ __forceinline__ unsigned lane_prefix(unsigned x, unsigned& total)
+{
+    const int lane = threadIdx.x & 63;
+    unsigned v = x;
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned o = __shfl_up(v, d, 64);
+        if (lane >= d) v += o;
+    }
+    total = __builtin_amdgcn_readlane(v, 63);
+    return v - x;
+}
+
+// A ticket of a launch that takes only some of its item ranges: the ranges
+// are visited 64 at a time (lane <-> range), each lane holding its range's
+// first item, its count (0: not taken) and the items of the taken ranges
+// before it in the chunk.  Tickets count taken items only, so no ticket is
+// spent on an item that is not computed; a wave's tickets only grow, so its
+// cursor only moves forward.
+struct RangeCursor {
+    int c0;            // first range of the current chunk
+    unsigned base;     // taken items before the chunk
+    unsigned total;    // taken items in the chunk
+    int first;         // this lane's range: first item
+    unsigned cnt, pre; // this lane's range: taken items, taken items before it in the chunk
+};
+
+// Item position of ticket t in the chunk that holds it: the lane whose range
+// covers it, and the item.
+__device__ __forceinline__ unsigned cursor_item(const RangeCursor& cur, unsigned t, int& lane_of)
+{
+    const unsigned r = t - cur.base;
+    const unsigned long long bal = __ballot(cur.cnt > 0u && r >= cur.pre && r < cur.pre + cur.cnt);
+    if (!bal) return 0xffffffffu;   // not reached: the chunk's ranges cover [base, base + total)
+    lane_of = __builtin_ctzll(bal);
+    const unsigned pre = __builtin_amdgcn_readlane(cur.pre, lane_of);
+    const int first = __builtin_amdgcn_readlane(cur.first, lane_of);
+    return (unsigned)first + (r - pre);
+}
+
+// Main list with skipped blocks: ranges are the (phase, block) segments in
+// list order, a segment taken unless its block is skipped.
+template <class R>
+__device__ __forceinline__ void main_chunk(const HfStepArgs<R>& a, RangeCursor& cur, int c0)
+{
+    const int k = c0 + (int)(threadIdx.x & 63);
+    const int nseg = 3 * a.nblocks;
+    cur.c0 = c0;
+    cur.first = 0;
+    cur.cnt = 0u;
+    if (k < nseg && !a.skip[k % a.nblocks]) {
+        cur.first = a.segs[2 * k];
+        cur.cnt = (unsigned)a.segs[2 * k + 1];
+    }
+    cur.pre = lane_prefix(cur.cnt, cur.total);
+}
+
+template <class M, int LIM, bool RESID>
+__device__ __forceinline__ void main_item(const HfStepArgs<typename M::R>& a, unsigned idx, int lane, int& cached,
+                                          double& c_step, double& c_pred, long long& c_kstep, long long& c_kpred)
+{
+    typedef typename M::R R;
+    const HfItem it = a.items[a.item0 + idx];
+    if (a.use_fixed) {
+        muscl_go<M, LIM, RESID>(a, it, lane, a.fixed_step, a.fixed_pred, a.imax_new);
+        return;
+    }
+    if (a.skip[it.block]) return;   // left to the guess launch
+    const HfBlockDev bd = a.blocks[it.block];
+    if (it.block != cached) {
+        cached = it.block;
+        step_dts<R>(a, bd, true, c_step, c_pred, c_kstep, c_kpred);
+    }
+    record_dts<R>(a, it, bd, true, c_step, c_pred, c_kstep, c_kpred);
+    muscl_go<M, LIM, RESID>(a, it, lane, c_step, c_pred, a.imax_new);
+}
+
 template <class M, int LIM, bool RESID>
 __global__ __launch_bounds__(256) void k_muscl(const HfStepArgs<typename M::R> a)
 {
-    typedef typename M::R R;
     const int lane = threadIdx.x & 63;
-    const unsigned nstatic = min(total_waves(), (unsigned)a.nmain);
-    // the block whose dts this wave has computed: speculative ones for the main
-    // list (c_fix false) or the fix-up verdict and exact ones (c_fix true)
-    int cached = -1;
-    bool c_fix = false, c_need = false;
+    int cached = -1;   // the block whose speculative dts this wave has computed
     double c_step = 0.0, c_pred = 0.0;
     long long c_kstep = 0, c_kpred = 0;
+    const unsigned nsk = (a.nskip && !a.use_fixed) ? __builtin_amdgcn_readfirstlane(*a.nskip) : 0u;
+    if (nsk == 0u) {   // every item of the list
+        const unsigned n = (unsigned)a.nmain;
+        const unsigned nstatic = min(total_waves(), n);
+        for (unsigned idx = first_ticket(a.sync, lane, nstatic, nstatic); idx < n;
+             idx = next_ticket(a.sync, lane, nstatic))
+            main_item<M, LIM, RESID>(a, idx, lane, cached, c_step, c_pred, c_kstep, c_kpred);
+        return;
+    }
+    // tickets over the segments of the blocks not skipped
+    const int nseg = 3 * a.nblocks;
+    unsigned n = 0u;
+    for (int c0 = 0; c0 < nseg; c0 += 64) {
+        const int k = c0 + lane;
+        const unsigned cnt = (k < nseg && !a.skip[k % a.nblocks]) ? (unsigned)a.segs[2 * k + 1] : 0u;
+        unsigned tot;
+        lane_prefix(cnt, tot);
+        n += tot;
+    }
+    const unsigned nstatic = min(total_waves(), n);
+    RangeCursor cur;
+    main_chunk<typename M::R>(a, cur, 0);
+    cur.base = 0u;
+    for (unsigned t = first_ticket(a.sync, lane, nstatic, nstatic); t < n; t = next_ticket(a.sync, lane, nstatic)) {
+        while (t >= cur.base + cur.total) {
+            const unsigned nb_ = cur.base + cur.total;
+            if (cur.c0 + 64 >= nseg) return;   // not reached: the chunks hold n items
+            main_chunk<typename M::R>(a, cur, cur.c0 + 64);
+            cur.base = nb_;
+        }
+        int j;
+        const unsigned idx = cursor_item(cur, t, j);
+        if (idx >= (unsigned)a.nmain) return;
+        main_item<M, LIM, RESID>(a, idx, lane, cached, c_step, c_pred, c_kstep, c_kpred);
+    }
+}
+
+// The guess launch's view of problem p: its plan and the cursor over its
+// blocks from the plan's first block on (a range per block: its items in the
+// fix-up rounds, taken if the block needs computing).
+struct GuessProblem {
+    int p, first, nb;
+    double O, m;
+    long long kstep;
+};
+
+template <class R>
+__device__ __forceinline__ bool guess_need(const HfStepArgs<R>& a, const GuessProblem& g, int c, int b)
+{
+    const double cf = cfl_eff(a.cfl, g.kstep + 1 + c);
+    return a.skip[b] != 0u || cf * g.O != cf * smin(1e16, g.m);
+}
+
+template <class R>
+__device__ __forceinline__ void guess_chunk(const HfStepArgs<R>& a, const GuessProblem& g, RangeCursor& cur, int c0)
+{
+    const int c = c0 + (int)(threadIdx.x & 63);
+    cur.c0 = c0;
+    cur.first = 0;
+    cur.cnt = 0u;
+    if (c >= g.first && c < g.nb) {
+        const int b = a.probs[g.p].first + c;
+        if (guess_need<R>(a, g, c, b)) {
+            cur.first = a.fixrange[2 * b];
+            cur.cnt = (unsigned)a.fixrange[2 * b + 1];
+        }
+    }
+    cur.pre = lane_prefix(cur.cnt, cur.total);
+}
+
+// The next problem from p on with blocks to compute (nprob if none).
+template <class R>
+__device__ __forceinline__ void guess_enter(const HfStepArgs<R>& a, GuessProblem& g, int p)
+{
+    for (; p < a.nprob; p++) {
+        const HfProbDev pr = a.probs[p];
+        if (pr.nb < 2) continue;
+        g.p = p;
+        g.nb = pr.nb;
+        g.first = guess_plan<R>(a, p, pr, g.O, g.m, g.kstep);
+        if (g.first < pr.nb) return;
+    }
+    g.p = a.nprob;
+}
+
+template <class M, int LIM>
+__global__ __launch_bounds__(256) void k_muscl_guess(const HfStepArgs<typename M::R> a)
+{
+    typedef typename M::R R;
+    const int lane = threadIdx.x & 63;
+    // items to compute, over every problem
+    unsigned n = 0u;
+    GuessProblem g;
+    for (guess_enter<R>(a, g, 0); g.p < a.nprob; guess_enter<R>(a, g, g.p + 1))
+        for (int c0 = g.first & ~63; c0 < g.nb; c0 += 64) {
+            RangeCursor cur;
+            guess_chunk<R>(a, g, cur, c0);
+            n += cur.total;
+        }
+    if (n == 0u) return;
+    const unsigned nstatic = min(total_waves(), n);
+    guess_enter<R>(a, g, 0);
+    RangeCursor cur;
+    guess_chunk<R>(a, g, cur, g.first & ~63);
+    cur.base = 0u;
+    int cb = -1;   // the block whose dts this wave holds
+    double dt_step = 0.0, dt_pred = 0.0;
+    long long kpred = 0;
+    for (unsigned t = first_ticket(a.sync, lane, nstatic, nstatic); t < n; t = next_ticket(a.sync, lane, nstatic)) {
+        while (t >= cur.base + cur.total) {
+            const unsigned nb_ = cur.base + cur.total;
+            int c0 = cur.c0 + 64;
+            if (c0 >= g.nb) {
+                guess_enter<R>(a, g, g.p + 1);
+                if (g.p >= a.nprob) return;   // not reached: the problems hold n items
+                c0 = g.first & ~63;
+            }
+            guess_chunk<R>(a, g, cur, c0);
+            cur.base = nb_;
+        }
+        int j;
+        const unsigned idx = cursor_item(cur, t, j);
+        if (idx < (unsigned)a.nmain || idx >= (unsigned)a.nitems) return;
+        const HfItem it = a.items[a.item0 + idx];
+        const HfBlockDev bd = a.blocks[it.block];
+        if (it.block != cb) {
+            cb = it.block;
+            kpred = g.kstep + 1 + bd.lb;
+            dt_step = cfl_eff(a.cfl, g.kstep) * g.O;
+            dt_pred = cfl_eff(a.cfl, kpred) * smin(1e16, g.m);
+            a.fixed[cb] = 1u;   // the same value from every wave
+        }
+        record_dts<R>(a, it, bd, false, dt_step, dt_pred, g.kstep, kpred);
+        muscl_go<M, LIM, false>(a, it, lane, dt_step, dt_pred, a.imax_fix);
+    }
+}
+
+template <class M, int LIM>
+__global__ __launch_bounds__(256) void k_muscl_check(const HfStepArgs<typename M::R> a)
+{
+    typedef typename M::R R;
+    const int lane = threadIdx.x & 63;
+    // the first round with a wrong block; wave 0 records the skip verdicts
+    int first = a.nrounds + 1;
+    const bool rec = wave_id() == 0;
+    for (int p = 0; p < a.nprob; p++) {
+        const HfProbDev pr = a.probs[p];
+        if (pr.nb < 2) continue;
+        double O, m;
+        long long kstep;
+        guess_plan<R>(a, p, pr, O, m, kstep);
+        const int w = plan_scan<R, PRE_GUESS>(a, pr, a.nblocks, kstep, O, m, rec);
+        first = w < first ? w : first;
+    }
+    if (first > a.nrounds) return;   // every block has its exact dt
+
+    unsigned long long* imax_chk = a.imax_fix + 2 * a.nblocks;
+    uint32_t* fixed_chk = a.fixed + a.nblocks;
+    const unsigned base = (unsigned)a.rounds[first];   // item of queue counter value 0
+    int cached = -1, cp = -1;   // the block whose verdict this wave holds, the problem of its guess minimum
+    bool c_need = false;
+    double cm = 0.0, c_step = 0.0, c_pred = 0.0;
+    long long c_kstep = 0, c_kpred = 0;
     int r_from = -1, r_busy = 0;   // first_busy_round(r_from) as this wave last computed it
-    for (unsigned idx = first_ticket(a.sync, lane, nstatic);; idx = next_ticket(a.sync, lane, nstatic)) {
+    for (unsigned idx = next_ticket(a.sync, lane, base);; idx = next_ticket(a.sync, lane, base)) {
         if (idx >= (unsigned)a.nitems) break;
         const HfItem it = a.items[a.item0 + idx];
-        if (a.use_fixed) {
-            muscl_go<M, LIM, RESID>(a, it, lane, a.fixed_step, a.fixed_pred, a.imax_new);
-            item_done(a.sync, lane);
-            continue;
-        }
-        const HfBlockDev bd = a.blocks[it.block];
-        if (idx < (unsigned)a.nmain) {
-            if (!a.skip[it.block]) {   // a skipped block is computed exactly in its fix-up round
-                if (it.block != cached || c_fix) {
-                    cached = it.block;
-                    c_fix = false;
-                    step_dts<R>(a, it.block, bd, true, DT_SPEC, c_step, c_pred, c_kstep, c_kpred);
-                }
-                record_dts<R>(a, it, bd, true, c_step, c_pred, c_kstep, c_kpred);
-                muscl_go<M, LIM, RESID>(a, it, lane, c_step, c_pred, a.imax_new);
-            }
-            item_done(a.sync, lane);
-            continue;
-        }
-        // fix-up round item
         const int l = round_of<R>(a, idx);
         if (l != r_from) {
-            if (!wait_done(a.sync, (unsigned)a.rounds[l])) {
-                atomicOr(&a.pending[(idx - a.nmain) >> 5], 1u << ((idx - a.nmain) & 31));
+            if (!wait_done(a.sync, (unsigned)a.rounds[l] - base)) {
+                const unsigned k = idx - (unsigned)a.nmain;
+                atomicOr(&a.pending[k >> 5], 1u << (k & 31));
                 continue;   // left to the last wave, not counted complete
             }
             r_from = l;
-            r_busy = first_busy_round<R>(a, l, lane, nstatic);
+            r_busy = first_busy_round<R>(a, l, lane, base);
         }
         if (r_busy == l) {
-            if (it.block != cached || !c_fix) {
+            const HfBlockDev bd = a.blocks[it.block];
+            if (it.block != cached) {
                 cached = it.block;
-                c_fix = true;
                 const HfProbDev pr = a.probs[bd.problem];
-                c_need = fix_verdict<R>(a, bd.problem, pr, bd.lb, c_step, c_pred, c_kstep, c_kpred);
+                if (bd.problem != cp) {
+                    cp = bd.problem;
+                    double O;
+                    long long ks;
+                    guess_plan<R>(a, cp, pr, O, cm, ks);
+                }
+                c_need = check_verdict<R>(a, bd.problem, pr, a.nblocks, bd.lb, cm, c_step, c_pred, c_kstep, c_kpred);
             }
             if (c_need) {
-                a.fixed[it.block] = 1u;
+                atomicExch(&fixed_chk[it.block], 1u);
                 record_dts<R>(a, it, bd, false, c_step, c_pred, c_kstep, c_kpred);
-                muscl_go<M, LIM, RESID>(a, it, lane, c_step, c_pred, a.imax_fix);
+                muscl_go<M, LIM, false>(a, it, lane, c_step, c_pred, imax_chk);
             }
         }
         item_done(a.sync, lane);
     }
     // The last wave to leave computes the items whose wait gave up, in queue
-    // order; every other wave has left, so all it reads is final.
+    // order; every other wave has left, so all it reads is final, and none of
+    // those items was written by another wave.
+    __builtin_amdgcn_s_waitcnt(0xF70);   // this wave's atomics are performed before it counts as gone
     const unsigned left = __builtin_amdgcn_readfirstlane(atomicAdd(&a.sync[HF_SYNC_EXITED], lane == 0 ? 1u : 0u));
     if (left != total_waves() - 1) return;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    if (!(__builtin_amdgcn_readfirstlane(a.sync[HF_SYNC_DONE]) & HF_SYNC_ABORT_BIT)) return;
+    if (!(__builtin_amdgcn_readfirstlane(atomic_load(&a.sync[HF_SYNC_DONE])) & HF_SYNC_ABORT_BIT)) return;
     const unsigned nfix = (unsigned)(a.nitems - a.nmain);
     for (unsigned w = 0; w * 32 < nfix; w++) {
-        const unsigned bits = __builtin_amdgcn_readfirstlane(a.pending[w]);
+        const unsigned bits = __builtin_amdgcn_readfirstlane(atomic_load(&a.pending[w]));
         for (unsigned k = 0; k < 32; k++) {
             if (!((bits >> k) & 1u)) continue;
             const unsigned idx = a.nmain + w * 32 + k;
             const HfItem it = a.items[a.item0 + idx];
             const HfBlockDev bd = a.blocks[it.block];
             const HfProbDev pr = a.probs[bd.problem];
-            double dt_step, dt_pred;
+            double O, mg, dt_step, dt_pred;
             long long kstep, kpred;
-            if (fix_verdict<R>(a, bd.problem, pr, bd.lb, dt_step, dt_pred, kstep, kpred)) {
-                a.fixed[it.block] = 1u;
+            guess_plan<R>(a, bd.problem, pr, O, mg, kstep);
+            if (check_verdict<R>(a, bd.problem, pr, a.nblocks, bd.lb, mg, dt_step, dt_pred, kstep, kpred)) {
+                atomicExch(&fixed_chk[it.block], 1u);
                 record_dts<R>(a, it, bd, false, dt_step, dt_pred, kstep, kpred);
-                muscl_go<M, LIM, RESID>(a, it, lane, dt_step, dt_pred, a.imax_fix);
-                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");   // its maxima before the next verdict
+                muscl_go<M, LIM, false>(a, it, lane, dt_step, dt_pred, imax_chk);
+                __builtin_amdgcn_s_waitcnt(0xF70);   // its maxima are performed before the next verdict reads them
             }
         }
-        a.pending[w] = 0u;
+        atomicExch(&a.pending[w], 0u);
     }
 }
 
@@ -601,7 +853,7 @@ __global__ __launch_bounds__(256) void k_godunov(const HfStepArgs<typename M::R>
     typedef typename M::R R;
     const int lane = threadIdx.x & 63;
     const unsigned nstatic = min(total_waves(), (unsigned)a.nitems);
-    for (unsigned idx = first_ticket(a.sync, lane, nstatic);; idx = next_ticket(a.sync, lane, nstatic)) {
+    for (unsigned idx = first_ticket(a.sync, lane, nstatic, nstatic);; idx = next_ticket(a.sync, lane, nstatic)) {
         if (idx >= (unsigned)a.nitems) break;
         const HfItem it = a.items[a.item0 + idx];
         double dt_step, dt_pred;
@@ -610,7 +862,7 @@ __global__ __launch_bounds__(256) void k_godunov(const HfStepArgs<typename M::R>
         } else {
             const HfBlockDev bd = a.blocks[it.block];
             long long kstep, kpred;
-            step_dts<R>(a, it.block, bd, false, DT_SPEC, dt_step, dt_pred, kstep, kpred);
+            step_dts<R>(a, bd, false, dt_step, dt_pred, kstep, kpred);
             const long long hs = kstep - a.hist_base;
             if (bd.problem == 0 && bd.lb == 0 && it.x0 == 0 && it.y0 == bd.g && hs >= 1 && hs <= a.hist_cap)
                 a.hist[hs - 1] = dt_step;
@@ -646,15 +898,28 @@ __global__ __launch_bounds__(256) void k_ghost(const HfGhost* __restrict__ tab, 
         }
         for (int i = threadIdx.x; i < book.nqueue; i += blockDim.x) book.queue[i] = 0u;
         for (int i = threadIdx.x; i < book.npending; i += blockDim.x) book.pending[i] = 0u;
-        for (int b = threadIdx.x; b < book.nblocks && book.fixed; b += blockDim.x) {
-            if (book.fixed[b]) {
-                book.imax_prev[2 * b] = book.imax_fix[2 * b];
-                book.imax_prev[2 * b + 1] = book.imax_fix[2 * b + 1];
+        __syncthreads();   // the skip count below starts from zero
+        const int nb = book.nblocks;
+        if (book.fixstats && book.fixed && threadIdx.x == 0) atomicAdd(&book.fixstats[0], 1ull);
+        for (int b = threadIdx.x; b < nb && book.fixed; b += blockDim.x) {
+            if (book.fixstats) {
+                if (book.skip[b]) atomicAdd(&book.fixstats[1], 1ull);
+                if (book.fixed[b]) atomicAdd(&book.fixstats[2], 1ull);
+                if (book.fixed[nb + b]) atomicAdd(&book.fixstats[3], 1ull);
+            }
+            // the check launch's recomputation over the guess launch's
+            const int s = book.fixed[nb + b] ? 1 : book.fixed[b] ? 0 : -1;
+            if (s >= 0) {
+                book.imax_prev[2 * b] = book.imax_fix[2 * (s * nb + b)];
+                book.imax_prev[2 * b + 1] = book.imax_fix[2 * (s * nb + b) + 1];
             }
             book.skip[b] = book.skip_next[b];
-            book.fixed[b] = 0u;
-            book.imax_fix[2 * b] = 0ull;
-            book.imax_fix[2 * b + 1] = 0ull;
+            if (book.skip_next[b]) atomicAdd(&book.queue[HF_SLOT_MAIN * HF_SYNC_WORDS + HF_SYNC_NSKIP], 1u);
+            for (int t = 0; t < 2; t++) {
+                book.fixed[t * nb + b] = 0u;
+                book.imax_fix[2 * (t * nb + b)] = 0ull;
+                book.imax_fix[2 * (t * nb + b) + 1] = 0ull;
+            }
         }
     }
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;

@@ -12,8 +12,8 @@ typedef HF_MATH Math;
 typedef Math::R Real;
 
 template <>
-hipError_t launch_update<Real>(int scheme, int limiter, bool resid, const HfStepArgs<Real>& a, int nwg, hipStream_t s,
-                              hipEvent_t e0, hipEvent_t e1)
+hipError_t launch_update<Real>(int scheme, int limiter, bool resid, int kind, const HfStepArgs<Real>& a, int nwg,
+                              hipStream_t s, hipEvent_t e0, hipEvent_t e1)
 {
     if (a.nitems <= 0) return hipSuccess;
     const dim3 block(256), grid(nwg);
@@ -25,6 +25,12 @@ hipError_t launch_update<Real>(int scheme, int limiter, bool resid, const HfStep
     if (scheme == 0) {
         if (resid) HF_GO((k_godunov<Math, true>));
         else HF_GO((k_godunov<Math, false>));
+    } else if (kind == HF_LAUNCH_GUESS) {
+        if (limiter == 0) HF_GO((k_muscl_guess<Math, 0>));
+        else HF_GO((k_muscl_guess<Math, 1>));
+    } else if (kind == HF_LAUNCH_CHECK) {
+        if (limiter == 0) HF_GO((k_muscl_check<Math, 0>));
+        else HF_GO((k_muscl_check<Math, 1>));
     } else if (limiter == 0) {
         if (resid) HF_GO((k_muscl<Math, 0, true>));
         else HF_GO((k_muscl<Math, 0, false>));

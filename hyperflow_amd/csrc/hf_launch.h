@@ -8,12 +8,16 @@
 namespace hf {
 
 
+// Update launches of a step: the main list, and for the MUSCL compat dt the
+// guess and check launches of the fix-up rounds.
+enum { HF_LAUNCH_MAIN = 0, HF_LAUNCH_GUESS = 1, HF_LAUNCH_CHECK = 2 };
+
 // e0/e1 (optional): events stamped at the dispatch's start and end by the
 // launch itself (hipExtLaunchKernel), for per-launch timing without extra
 // commands on the stream.
 template <class R>
-hipError_t launch_update(int scheme, int limiter, bool resid, const HfStepArgs<R>& a, int grid, hipStream_t s,
-                         hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+hipError_t launch_update(int scheme, int limiter, bool resid, int kind, const HfStepArgs<R>& a, int grid,
+                         hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // Workgroups (256 threads) of an update kernel that fit on one CU at once.
 template <class R>
 int update_wgs_per_cu(int scheme, int limiter, bool resid);

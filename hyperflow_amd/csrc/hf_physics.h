@@ -508,30 +508,15 @@ __device__ __forceinline__ double cfl_eff(double cfl, long long call)
     return (call <= 10) ? 0.2 * cfl : cfl;   // timestep.cpp:60-65,72,75
 }
 
-// Largest wave speeds of the new interior of block c after the step's
-// fix-up launches so far.
-// New maxima of block c as seen by a predictor dt: none (every block at U^n,
-// the speculative value) or final (after the main launch and the fix-ups so
-// far).
-enum { DT_SPEC = 0, DT_FINAL = 2 };
-
+// Step dt and speculative predictor dt of a block: every block of the
+// problem at U^n.  The reference's compat predictor dt (simulation.cpp:106-110
+// with muscl_hancock.cpp:246) sees the blocks before this one at U^{n+1}; the
+// fix-up launches below compute it and recompute a block whenever the two
+// differ.  In uniform mode (or for Godunov, which has no predictor) the
+// predictor dt is the step dt.
 template <class R>
-__device__ __forceinline__ double new_max(const HfStepArgs<R>& a, int c, int axis, int view)
-{
-    // written by earlier launches only
-    return bitsd((view == DT_FINAL && a.fixed[c]) ? a.imax_fix[2 * c + axis] : a.imax_new[2 * c + axis]);
-}
-
-// Step dt and predictor dt of a block.  With view DT_FINAL the predictor dt
-// follows the reference's compat semantics: blocks of the
-// problem before this one are already at U^{n+1} (simulation.cpp:106-110 with
-// muscl_hancock.cpp:246).  With DT_SPEC every block is taken at U^n, the
-// speculative value of the main launch; a fix-up launch recomputes a block
-// whenever the dt it used differs from the DT_FINAL one.
-template <class R>
-__device__ __forceinline__ void step_dts(const HfStepArgs<R>& a, int block, const HfBlockDev& bd, bool muscl,
-                                         int view, double& dt_step, double& dt_pred, long long& kstep,
-                                         long long& kpred)
+__device__ __forceinline__ void step_dts(const HfStepArgs<R>& a, const HfBlockDev& bd, bool muscl, double& dt_step,
+                                         double& dt_pred, long long& kstep, long long& kpred)
 {
     // Lane l takes blocks first+l, first+l+64, ...: the per-block loads and
     // divisions run side by side instead of as one dependent chain, then the
@@ -541,81 +526,188 @@ __device__ __forceinline__ void step_dts(const HfStepArgs<R>& a, int block, cons
     const HfProbDev pr = a.probs[bd.problem];
     kstep = (long long)a.calls[bd.problem] - pr.cps + 1;
     kpred = kstep + 1 + bd.lb;
-    double dtx = 1e16, dty = 1e16, ptx = 1e16, pty = 1e16;
-    const bool compat = muscl && a.dt_mode == 0;
+    double dtx = 1e16, dty = 1e16;
     const int end = pr.first + pr.nb;
     for (int c = pr.first + (int)(threadIdx.x & 63); c < end; c += 64) {
         const double cdx = a.blocks[c].dx, cdy = a.blocks[c].dy;
         const double gx = bitsd(a.gmax[2 * c]), gy = bitsd(a.gmax[2 * c + 1]);
         const double ox = smax(bitsd(a.imax_old[2 * c]), gx), oy = smax(bitsd(a.imax_old[2 * c + 1]), gy);
-        const double sx = cdx / ox, sy = cdy / oy;
-        dtx = smin(dtx, sx);
-        dty = smin(dty, sy);
-        if (compat) {
-            if (view != DT_SPEC && c < block) {
-                ptx = smin(ptx, cdx / smax(new_max(a, c, 0, view), gx));
-                pty = smin(pty, cdy / smax(new_max(a, c, 1, view), gy));
-            } else {
-                ptx = smin(ptx, sx);
-                pty = smin(pty, sy);
-            }
-        }
+        dtx = smin(dtx, cdx / ox);
+        dty = smin(dty, cdy / oy);
     }
     dtx = wave_min(dtx);
     dty = wave_min(dty);
     dt_step = cfl_eff(a.cfl, kstep) * smin(dtx, dty);
-    if (compat) {
-        ptx = wave_min(ptx);
-        pty = wave_min(pty);
-        dt_pred = cfl_eff(a.cfl, kpred) * smin(ptx, pty);
-    } else {
-        dt_pred = dt_step;
-    }
+    dt_pred = (muscl && a.dt_mode == 0) ? cfl_eff(a.cfl, kpred) * smin(dtx, dty) : dt_step;
 }
 
-// step_dts of a compat MUSCL block in both views from one pass over the
-// problem's blocks: the step dt, the speculative predictor dt (DT_SPEC: every
-// block at U^n, i.e. the step's own minima) and the exact one (DT_FINAL).
-// The same operands in the same order as two step_dts calls, so the same bits.
-// The block is the lb-th of problem p (record pr): no block record is needed,
-// and every load of the pass depends only on pr, so they issue together.
+// ------------------------------------------------------ fix-up planning --
+// The compat predictor dt of block l of a problem is cfl_eff(k_l) times
+//     min(1e16, min over c < l of pre_c, min over c >= l of old_c)
+// with old_c = min(dx_c / max(old x maxima, x ghosts), dy_c / max(old y
+// maxima, y ghosts)) (block c at U^n) and pre_c the same with block c's final
+// U^{n+1} maxima: a minimum over one set of non-NaN terms, so any grouping
+// gives the reference's bits (step_dts takes the x and y minima apart and
+// then their minimum, the same set).  The speculative dt of the main list is
+// cfl_eff(k_l) * min(1e16, min over all c of old_c); the step dt is
+// cfl_eff(k_step) times the same minimum.
+
 template <class R>
-__device__ __forceinline__ void step_dts_fix(const HfStepArgs<R>& a, int p, const HfProbDev& pr, int lb,
-                                             double& dt_step, double& spec, double& exact, long long& kstep,
-                                             long long& kpred)
+__device__ __forceinline__ double term_old(const HfStepArgs<R>& a, int c)
+{
+    const double cdx = a.blocks[c].dx, cdy = a.blocks[c].dy;
+    const double gx = bitsd(a.gmax[2 * c]), gy = bitsd(a.gmax[2 * c + 1]);
+    return smin(cdx / smax(bitsd(a.imax_old[2 * c]), gx), cdy / smax(bitsd(a.imax_old[2 * c + 1]), gy));
+}
+
+// pre_c from the given new maxima bit patterns
+template <class R>
+__device__ __forceinline__ double term_new(const HfStepArgs<R>& a, int c, unsigned long long mx, unsigned long long my)
+{
+    const double cdx = a.blocks[c].dx, cdy = a.blocks[c].dy;
+    const double gx = bitsd(a.gmax[2 * c]), gy = bitsd(a.gmax[2 * c + 1]);
+    return smin(cdx / smax(bitsd(mx), gx), cdy / smax(bitsd(my), gy));
+}
+
+// Which maxima are final for pre_c:
+//  PRE_MAIN  - the main launch's (at the start of the guess launch nothing has
+//              been recomputed);
+//  PRE_GUESS - the guess launch's where it recomputed the block, else the
+//              main launch's (at the start of the check launch);
+//  PRE_CHECK - in the check launch, also the blocks it has recomputed so far:
+//              written by other waves of the same launch, so the check
+//              launch's slot is read with device-scope atomic loads.
+enum { PRE_MAIN = 0, PRE_GUESS = 1, PRE_CHECK = 2 };
+
+template <class R, int SRC>
+__device__ __forceinline__ double term_pre(const HfStepArgs<R>& a, int nb_all, int c)
+{
+    const unsigned long long* fix1 = a.imax_fix + 2 * nb_all;
+    if (SRC == PRE_CHECK) {
+        const bool f2 = __hip_atomic_load(&a.fixed[nb_all + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+        const unsigned long long x2 = __hip_atomic_load(&fix1[2 * c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long y2 = __hip_atomic_load(&fix1[2 * c + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (f2) return term_new(a, c, x2, y2);
+    }
+    if (SRC != PRE_MAIN && a.fixed[c]) return term_new(a, c, a.imax_fix[2 * c], a.imax_fix[2 * c + 1]);
+    return term_new(a, c, a.imax_new[2 * c], a.imax_new[2 * c + 1]);
+}
+
+// Inclusive minimum over lanes 0..lane (scan_lo) or lane..63 (scan_hi).
+__device__ __forceinline__ double scan_min_lo(double x)
+{
+    const int lane = threadIdx.x & 63;
+    for (int d = 1; d < 64; d <<= 1) {
+        const double o = __shfl_up(x, d, 64);
+        if (lane >= d) x = (o < x) ? o : x;
+    }
+    return x;
+}
+__device__ __forceinline__ double scan_min_hi(double x)
+{
+    const int lane = threadIdx.x & 63;
+    for (int d = 1; d < 64; d <<= 1) {
+        const double o = __shfl_down(x, d, 64);
+        if (lane + d < 64) x = (o < x) ? o : x;
+    }
+    return x;
+}
+
+// The fix-up plan of one problem from maxima written by earlier launches
+// (plain loads), in one pass over its blocks, 64 at a time (lane <-> block):
+// suffix minima of old_c and prefix minima of pre_c give every block's
+// predictor dt at once.
+//  GUESS (SRC = PRE_MAIN): returns the first block l >= 1 the main list got
+//    wrong (skipped, or its speculative dt differs from the exact one) and in
+//    m that block's minimum; nb if none.  The guess launch computes every
+//    block from there on with cfl_eff(k_l) * min(1e16, m), the exact dt in
+//    case no block it recomputes sets a new minimum, and needs to touch
+//    block l >= first only if that differs from its speculative dt or the
+//    main list skipped it.
+//  CHECK (SRC = PRE_GUESS, m = the guess minimum): returns the first block
+//    l >= 1 whose computed dt is not the exact one, nb if none, and records
+//    the next step's skip verdicts (exact != speculative) of the blocks up to
+//    it when `record`.
+// O: the problem's speculative minimum min(1e16, min over c of old_c).
+template <class R, int SRC>
+__device__ int plan_scan(const HfStepArgs<R>& a, const HfProbDev& pr, int nb_all, long long kstep, double O,
+                         double& m, bool record)
+{
+    const int lane = threadIdx.x & 63;
+    const int nb = pr.nb;
+    const double inf = __builtin_huge_val();
+    double carry = inf;   // minimum of pre_c over the chunks before this one
+    for (int c0 = 0; c0 < nb; c0 += 64) {
+        const int c = c0 + lane;
+        const bool in = c < nb;
+        const int b = pr.first + (in ? c : 0);
+        double o = inf, n = inf;
+        if (in) {
+            o = term_old(a, b);
+            n = term_pre<R, SRC>(a, nb_all, b);
+        }
+        double tail = inf;   // minimum of old_c after this chunk
+        for (int t = c0 + 64 + lane; t < nb; t += 64) tail = smin(tail, term_old(a, pr.first + t));
+        tail = wave_min(tail);
+        double pre = __shfl_up(scan_min_lo(n), 1, 64);
+        pre = (lane == 0 || pre > carry) ? carry : pre;
+        double suf = scan_min_hi(o);
+        suf = (tail < suf) ? tail : suf;
+        const double e = (pre < suf) ? pre : suf;
+        const double cf = cfl_eff(a.cfl, kstep + 1 + c);
+        const double exact = cf * smin(1e16, e);
+        const double spec = cf * smin(1e16, O);
+        bool busy;
+        if (SRC == PRE_MAIN) {
+            busy = in && c >= 1 && (a.skip[b] != 0u || spec != exact);
+        } else {
+            const bool g = a.fixed[b] != 0u;
+            const double used = g ? cf * smin(1e16, m) : spec;
+            busy = in && c >= 1 && ((!g && a.skip[b] != 0u) || used != exact);
+        }
+        const unsigned long long bal = __ballot(busy);
+        const int first = bal ? __builtin_ctzll(bal) : 64;
+        if (SRC != PRE_MAIN && record && in && c >= 1 && lane <= first) a.skip_next[b] = spec != exact ? 1u : 0u;
+        if (bal) {
+            if (SRC == PRE_MAIN) m = __shfl(e, first, 64);
+            return c0 + first;
+        }
+        carry = smin(carry, wave_min(n));
+    }
+    return nb;
+}
+
+// The check launch's verdict on block lb of problem p once every block
+// before it is final (the launch has completed every earlier round): the
+// step dt, the exact predictor dt and whether the block must be (re)computed
+// -- its dt so far (the guess launch's, with guess minimum mg, or the main
+// list's speculative one) is not the exact one.  Records the next step's skip
+// verdict.
+template <class R>
+__device__ __forceinline__ bool check_verdict(const HfStepArgs<R>& a, int p, const HfProbDev& pr, int nb_all, int lb,
+                                              double mg, double& dt_step, double& dt_pred, long long& kstep,
+                                              long long& kpred)
 {
     const int block = pr.first + lb;
     kstep = (long long)a.calls[p] - pr.cps + 1;
     kpred = kstep + 1 + lb;
-    double dtx = 1e16, dty = 1e16, ptx = 1e16, pty = 1e16;
+    double O = 1e16, P = 1e16;
     const int end = pr.first + pr.nb;
     for (int c = pr.first + (int)(threadIdx.x & 63); c < end; c += 64) {
-        const double cdx = a.blocks[c].dx, cdy = a.blocks[c].dy;
-        const double gx = bitsd(a.gmax[2 * c]), gy = bitsd(a.gmax[2 * c + 1]);
-        const double ox = smax(bitsd(a.imax_old[2 * c]), gx), oy = smax(bitsd(a.imax_old[2 * c + 1]), gy);
-        // final maxima of the blocks before this one (new_max with DT_FINAL),
-        // both candidates loaded at once
-        const bool fx = a.fixed[c] != 0u;
-        const unsigned long long fxx = a.imax_fix[2 * c], fxy = a.imax_fix[2 * c + 1];
-        const unsigned long long nwx = a.imax_new[2 * c], nwy = a.imax_new[2 * c + 1];
-        const double sx = cdx / ox, sy = cdy / oy;
-        dtx = smin(dtx, sx);
-        dty = smin(dty, sy);
-        if (c < block) {
-            ptx = smin(ptx, cdx / smax(bitsd(fx ? fxx : nwx), gx));
-            pty = smin(pty, cdy / smax(bitsd(fx ? fxy : nwy), gy));
-        } else {
-            ptx = smin(ptx, sx);
-            pty = smin(pty, sy);
-        }
+        const double o = term_old(a, c);
+        O = smin(O, o);
+        P = smin(P, c < block ? term_pre<R, PRE_CHECK>(a, nb_all, c) : o);
     }
-    dtx = wave_min(dtx);
-    dty = wave_min(dty);
-    ptx = wave_min(ptx);
-    pty = wave_min(pty);
-    dt_step = cfl_eff(a.cfl, kstep) * smin(dtx, dty);
-    spec = cfl_eff(a.cfl, kpred) * smin(dtx, dty);
-    exact = cfl_eff(a.cfl, kpred) * smin(ptx, pty);
+    O = wave_min(O);
+    P = wave_min(P);
+    const double cf = cfl_eff(a.cfl, kpred);
+    dt_step = cfl_eff(a.cfl, kstep) * O;
+    dt_pred = cf * P;
+    const double spec = cf * O;
+    const bool g = a.fixed[block] != 0u;
+    const double used = g ? cf * smin(1e16, mg) : spec;
+    a.skip_next[block] = spec != dt_pred ? 1u : 0u;   // the same value from every wave
+    return (!g && a.skip[block] != 0u) || used != dt_pred;
 }
 
 }  // namespace hf

@@ -28,7 +28,7 @@ HF_RUN_GRAPH = 1
 
 EXPORTS = ("hf_create", "hf_destroy", "hf_ghost_cells", "hf_upload", "hf_download", "hf_apply_bcs",
            "hf_time_step", "hf_scheme_residual", "hf_advance", "hf_run", "hf_synchronize", "hf_dt_history",
-           "hf_calls", "hf_profile", "hf_kernel_stats", "hf_conservation", "hf_step_noalloc", "hf_last_error",
+           "hf_calls", "hf_profile", "hf_kernel_stats", "hf_fixup_stats", "hf_conservation", "hf_step_noalloc", "hf_last_error",
            "hf_version")
 # include/hyperflow_learned.h
 EXPORTS_LEARNED = ("hf_net_create", "hf_net_destroy", "hf_net_set_stream", "hf_net_set_desc", "hf_net_param_count",
@@ -86,6 +86,7 @@ def lib():
         L.hf_calls.argtypes = [vp, C.c_int, C.POINTER(C.c_int64)]
         L.hf_profile.argtypes = [vp, C.c_int]
         L.hf_kernel_stats.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+        L.hf_fixup_stats.argtypes = [vp, C.POINTER(C.c_int64)]
         L.hf_conservation.argtypes = [vp, C.c_int, _dp]
         L.hf_step_noalloc.argtypes = [C.POINTER(hf_case_desc), vp, vp, C.POINTER(C.c_int64), vp, vp]
         L.hf_last_error.restype = C.c_char_p
@@ -291,6 +292,12 @@ class Solver:
     def profile(self, enable, every: int = 1) -> None:
         """Time the update launches of every `every`-th step (off if not enable)."""
         _check(lib().hf_profile(self._h, int(every) if enable else 0), "hf_profile")
+
+    def fixup_stats(self) -> dict:
+        """Fix-up tallies of the compat MUSCL dt since the last call."""
+        out = (C.c_int64 * 4)()
+        _check(lib().hf_fixup_stats(self._h, out), "hf_fixup_stats")
+        return {"steps": out[0], "skipped": out[1], "guess": out[2], "check": out[3]}
 
     def kernel_stats(self):
         n = C.c_int64()

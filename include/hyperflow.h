@@ -143,6 +143,13 @@ HF_API int hf_calls(hf_ctx* ctx, int problem, int64_t* calls);
  * timed launches and their summed duration. */
 HF_API int hf_profile(hf_ctx* ctx, int enable);
 HF_API int hf_kernel_stats(hf_ctx* ctx, int64_t* launches, double* total_ms);
+/* Fix-up tallies of the reference-compatible MUSCL dt since the last call
+ * (counted by each following step): out[0] steps, out[1] blocks the main
+ * launch left to the guess launch, out[2] blocks the guess launch computed,
+ * out[3] blocks the check launch recomputed.  No counterpart in the
+ * reference: it describes how this implementation meets the reference's
+ * sequential block order (simulation.cpp:106-110). */
+HF_API int hf_fixup_stats(hf_ctx* ctx, int64_t out[4]);
 /* Compensated (Neumaier) FP64 sums of rho, rho*u, rho*v, E over the interior
  * cells of one problem, reduced on the device in a fixed order. */
 HF_API int hf_conservation(hf_ctx* ctx, int problem, double sums[4]);
