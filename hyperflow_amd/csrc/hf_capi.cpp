@@ -275,8 +275,10 @@ HfStepArgs<R> step_args(hf_ctx* c, int item0, int nitems, const void* A, const v
     a.segs = c->d_rounds + segs_offset(*c);
     a.fixrange = c->d_rounds + fixrange_offset(*c);
     a.nskip = nullptr;
+    a.wait_ticks = c->wait_ticks;
     a.nrounds = 0;
     a.pending = c->d_pending;
+    a.npending = c->npending;
     a.imax_fix = c->d_imax_fix;
     a.fixed = c->d_fixed;
     a.skip = c->d_skip;
@@ -624,6 +626,12 @@ int build_plan(hf_ctx* c, const hf_case_desc* desc, bool dense)
         const int per_cu = c->prec == HF_FP64 ? hf::update_wgs_per_cu<double>(c->scheme, c->limiter, false)
                                               : hf::update_wgs_per_cu<float>(c->scheme, c->limiter, false);
         c->wg_cap = std::max(1, cus * per_cu);
+        // in-launch waits give up after 10 ms of the counter kernels read
+        // with s_memrealtime (its rate in kHz)
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+            khz = 100000;
+        c->wait_ticks = (unsigned long long)khz * 10ull;
     }
     const int64_t wave_slots = 4LL * c->wg_cap;
 

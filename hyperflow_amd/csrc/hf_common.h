@@ -65,14 +65,15 @@ struct HfStepArgs {
     double fixed_step, fixed_pred;
     uint32_t* sync;                      // HF_SYNC_WORDS counters of this launch (zero at launch)
     const int32_t* rounds;               // rounds[l]: first item of fix-up round l (1..nrounds); rounds[nrounds+1] = nitems
-    int32_t nrounds, pad1;
+    int32_t nrounds, npending;
     // Item ranges (first, count): segs[2 * (phase * nblocks + b)] of block b
     // in each of the main list's three phases, fixrange[2 * b] of block b in
     // the fix-up rounds; nskip: blocks the main list skips this step.
     const int32_t* segs;
     const int32_t* fixrange;
     const uint32_t* nskip;
-    uint32_t* pending;                   // bitmap over the fix-up items left to the check launch's last wave
+    unsigned long long wait_ticks;       // bound of an in-launch wait in s_memrealtime ticks (10 ms)
+    uint32_t* pending;                   // [npending] bitmap over the fix-up items left to the check launch's last wave
     // Fix-up state of the step.  Slot 0 belongs to the guess launch, slot 1 to
     // the check launch: imax_fix[s * 2 * nblocks + 2 * b + axis] holds the
     // maxima of block b recomputed by launch s, fixed[s * nblocks + b] says
@@ -105,7 +106,7 @@ struct GhostBook {
     int32_t nclear;               // elements in each
     uint32_t* queue;              // sync counters of this step's update launches (HF_SLOTS x HF_SYNC_WORDS)
     int32_t nqueue;
-    uint32_t* pending;            // pending bitmap of this step's update launch
+    uint32_t* pending;            // pending bitmap of this step's check launch
     int32_t npending;             // words
     int32_t nblocks;
     // fold the previous step's fix-ups into its maxima (the check launch's

@@ -43,6 +43,7 @@ struct hf_ctx {
     int main_begin = 0, main_items = 0;           // item range of a step's main launch
     int maxnb = 1;
     int wg_cap = 1024;                            // resident workgroups of an update launch
+    unsigned long long wait_ticks = 1000000;      // bound of an in-launch wait (s_memrealtime ticks)
     int64_t total = 0;                            // elements per state buffer
     void* buf[2] = {nullptr, nullptr};
     int cur = 0;

@@ -79,18 +79,11 @@ __device__ __forceinline__ unsigned first_ticket(uint32_t* sync, int lane, unsig
 }
 
 // What waves of one launch tell each other travels only through device-scope
-// atomics (wave-speed maxima, fixed flags, the pending bitmap, the counters),
-// which are coherent across the XCDs' L2 caches without cache maintenance:
-// writers update with atomic read-modify-writes, readers use atomic loads.
-// The new states themselves are not read inside the launch.
-__device__ __forceinline__ unsigned atomic_load(const uint32_t* p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ unsigned long long atomic_load(const unsigned long long* p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+// atomics (wave-speed maxima, fixed flags, the pending bitmap, the counters):
+// writers update with atomic read-modify-writes, and readers of values that
+// decide results read with coherent_read (hf_physics.h), an atomic
+// read-modify-write that changes nothing.  The new states themselves are not
+// read inside the launch.
 
 // One item of the launch is complete.  This wave's atomics (its maxima and
 // fixed flag) must be performed before the count that waiting waves poll:
@@ -106,19 +99,21 @@ __device__ __forceinline__ void item_done(uint32_t* sync, int lane, unsigned n =
 // Bounded wait until `target` items of the launch are complete (every item
 // before position `target` of the queue: items are handed out in order and an
 // item that waits never completes before the items it waits for).  The wait
-// gives up after WAIT_TICKS of the 100 MHz real-time counter, or at once when
+// gives up after wait_ticks of the real-time counter (s_memrealtime, the
+// clock HIP reports as hipDeviceAttributeWallClockRate), or at once when
 // another wave has given up, and then returns false with the abort bit set:
 // the caller leaves its item to the launch's last wave.  One load per poll
-// (count and abort bit share a word), about 0.85 us apart.
-constexpr unsigned long long WAIT_TICKS = 1000000ull;   // 10 ms
-__device__ __forceinline__ bool wait_done(uint32_t* sync, unsigned target)
+// (count and abort bit share a word); a load served late only delays the
+// wait.
+__device__ __forceinline__ bool wait_done(uint32_t* sync, unsigned target, unsigned long long wait_ticks)
 {
     const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
     for (;;) {
-        const unsigned d = __builtin_amdgcn_readfirstlane(atomic_load(&sync[HF_SYNC_DONE]));
+        const unsigned d = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(&sync[HF_SYNC_DONE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
         if (d & HF_SYNC_ABORT_BIT) return false;
         if (d >= target) return true;
-        if (__builtin_amdgcn_s_memrealtime() - t0 > WAIT_TICKS) {
+        if (__builtin_amdgcn_s_memrealtime() - t0 > wait_ticks) {
             __hip_atomic_fetch_or(&sync[HF_SYNC_DONE], HF_SYNC_ABORT_BIT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return false;
         }
@@ -633,20 +628,25 @@ __global__ __launch_bounds__(256) void k_muscl_guess(const HfStepArgs<typename M
 {
     typedef typename M::R R;
     const int lane = threadIdx.x & 63;
-    // items to compute, over every problem
+    // items to compute, over every problem; the walk below starts from the
+    // first problem's first chunk, kept from this pass
     unsigned n = 0u;
-    GuessProblem g;
+    GuessProblem g, g0;
+    RangeCursor cur;
+    g0.p = a.nprob;
     for (guess_enter<R>(a, g, 0); g.p < a.nprob; guess_enter<R>(a, g, g.p + 1))
         for (int c0 = g.first & ~63; c0 < g.nb; c0 += 64) {
-            RangeCursor cur;
-            guess_chunk<R>(a, g, cur, c0);
-            n += cur.total;
+            RangeCursor ck;
+            guess_chunk<R>(a, g, ck, c0);
+            if (g0.p == a.nprob) {
+                g0 = g;
+                cur = ck;
+            }
+            n += ck.total;
         }
     if (n == 0u) return;
     const unsigned nstatic = min(total_waves(), n);
-    guess_enter<R>(a, g, 0);
-    RangeCursor cur;
-    guess_chunk<R>(a, g, cur, g.first & ~63);
+    g = g0;
     cur.base = 0u;
     int cb = -1;   // the block whose dts this wave holds
     double dt_step = 0.0, dt_pred = 0.0;
@@ -712,7 +712,7 @@ __global__ __launch_bounds__(256) void k_muscl_check(const HfStepArgs<typename M
         const HfItem it = a.items[a.item0 + idx];
         const int l = round_of<R>(a, idx);
         if (l != r_from) {
-            if (!wait_done(a.sync, (unsigned)a.rounds[l] - base)) {
+            if (!wait_done(a.sync, (unsigned)a.rounds[l] - base, a.wait_ticks)) {
                 const unsigned k = idx - (unsigned)a.nmain;
                 atomicOr(&a.pending[k >> 5], 1u << (k & 31));
                 continue;   // left to the last wave, not counted complete
@@ -747,10 +747,10 @@ __global__ __launch_bounds__(256) void k_muscl_check(const HfStepArgs<typename M
     __builtin_amdgcn_s_waitcnt(0xF70);   // this wave's atomics are performed before it counts as gone
     const unsigned left = __builtin_amdgcn_readfirstlane(atomicAdd(&a.sync[HF_SYNC_EXITED], lane == 0 ? 1u : 0u));
     if (left != total_waves() - 1) return;
-    if (!(__builtin_amdgcn_readfirstlane(atomic_load(&a.sync[HF_SYNC_DONE])) & HF_SYNC_ABORT_BIT)) return;
+    if (!(__builtin_amdgcn_readfirstlane(coherent_read(&a.sync[HF_SYNC_DONE])) & HF_SYNC_ABORT_BIT)) return;
     const unsigned nfix = (unsigned)(a.nitems - a.nmain);
     for (unsigned w = 0; w * 32 < nfix; w++) {
-        const unsigned bits = __builtin_amdgcn_readfirstlane(atomic_load(&a.pending[w]));
+        const unsigned bits = __builtin_amdgcn_readfirstlane(coherent_read(&a.pending[w]));
         for (unsigned k = 0; k < 32; k++) {
             if (!((bits >> k) & 1u)) continue;
             const unsigned idx = a.nmain + w * 32 + k;

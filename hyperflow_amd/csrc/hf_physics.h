@@ -576,17 +576,31 @@ __device__ __forceinline__ double term_new(const HfStepArgs<R>& a, int c, unsign
 //              main launch's (at the start of the check launch);
 //  PRE_CHECK - in the check launch, also the blocks it has recomputed so far:
 //              written by other waves of the same launch, so the check
-//              launch's slot is read with device-scope atomic loads.
+//              launch's slot is read with coherent_read.
 enum { PRE_MAIN = 0, PRE_GUESS = 1, PRE_CHECK = 2 };
+
+// A word that other waves of the same launch update with device-scope
+// atomics, read with a device-scope atomic read-modify-write that changes
+// nothing: it is performed where those atomics are, so it sees every one of
+// them that has completed.  A plain atomic load may be served by the reading
+// XCD's own L2 from a line cached before those updates.
+__device__ __forceinline__ unsigned coherent_read(uint32_t* p)
+{
+    return __hip_atomic_fetch_or(p, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long coherent_read(unsigned long long* p)
+{
+    return __hip_atomic_fetch_or(p, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 template <class R, int SRC>
 __device__ __forceinline__ double term_pre(const HfStepArgs<R>& a, int nb_all, int c)
 {
-    const unsigned long long* fix1 = a.imax_fix + 2 * nb_all;
+    unsigned long long* fix1 = a.imax_fix + 2 * nb_all;
     if (SRC == PRE_CHECK) {
-        const bool f2 = __hip_atomic_load(&a.fixed[nb_all + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
-        const unsigned long long x2 = __hip_atomic_load(&fix1[2 * c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned long long y2 = __hip_atomic_load(&fix1[2 * c + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const bool f2 = coherent_read(&a.fixed[nb_all + c]) != 0u;
+        const unsigned long long x2 = coherent_read(&fix1[2 * c]);
+        const unsigned long long y2 = coherent_read(&fix1[2 * c + 1]);
         if (f2) return term_new(a, c, x2, y2);
     }
     if (SRC != PRE_MAIN && a.fixed[c]) return term_new(a, c, a.imax_fix[2 * c], a.imax_fix[2 * c + 1]);
@@ -667,9 +681,9 @@ __device__ int plan_scan(const HfStepArgs<R>& a, const HfProbDev& pr, int nb_all
         }
         const unsigned long long bal = __ballot(busy);
         const int first = bal ? __builtin_ctzll(bal) : 64;
-        if (SRC != PRE_MAIN && record && in && c >= 1 && lane <= first) a.skip_next[b] = spec != exact ? 1u : 0u;
+        if (SRC == PRE_GUESS && record && in && c >= 1 && lane <= first) a.skip_next[b] = spec != exact ? 1u : 0u;
         if (bal) {
-            if (SRC == PRE_MAIN) m = __shfl(e, first, 64);
+            if (SRC != PRE_GUESS) m = __shfl(e, first, 64);
             return c0 + first;
         }
         carry = smin(carry, wave_min(n));
