@@ -6,7 +6,10 @@ synthetic initial data (the inflow state), one MI355X.
 
 A step is one full time step of the path: ghost fill, CFL timestep reduction
 and the eight fused per-block MUSCL-HLLC updates.  The timed region starts
-with the state resident in HBM.  Prints one JSON line.
+with the state resident in HBM.  Prints one JSON line; its `learned`
+sub-object is the learned correction's speed-up at equal density L1 on one
+unseen seed (`--workload learned` below, about a minute, in a child process;
+--no-learned leaves it out).
 
 Other workloads of the same path (not the headline line):
     --workload godunov-quadrants
@@ -121,17 +124,21 @@ def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 20):
             "launches": reps + 3}
 
 
-def learned_line(seeds=(5000, 5001, 5002), coarse_steps: int = 200):
+LEARNED_RESOLUTIONS = (1024, 1152, 1280, 1536, 1792)
+
+
+def learned_line(seeds=(5000, 5001, 5002), coarse_steps: int = 200, resolutions=LEARNED_RESOLUTIONS):
     """The learned-correction comparison on unseen seeds: density L1 against
     the 2x2-averaged 2048^2 solution, and wall times (HIP graphs, state in
-    HBM), for plain runs at 1024..1792^2 and the corrected 1024^2 run."""
+    HBM), for plain runs at the given resolutions and the corrected 1024^2
+    run."""
     from hyperflow_amd import learned as L
-    L.ensure_tuned((1024, 1152, 1280, 1536, 1792, 2048), log=lambda m: print(m, file=sys.stderr))
+    L.ensure_tuned(tuple(resolutions) + (2048,), log=lambda m: print(m, file=sys.stderr))
     net = L.load()
     meta = L.load_meta()
     every, start = int(meta["every"]), int(meta["start"])
-    rows, summ = L.evaluate(net, list(seeds), coarse_steps, every=(every,), start=start,
-                            log=lambda m: print(m, file=sys.stderr))
+    rows, summ = L.evaluate(net, list(seeds), coarse_steps, resolutions=tuple(resolutions), every=(every,),
+                            start=start, log=lambda m: print(m, file=sys.stderr))
     lr = summ["learned"][every]
     return {"metric": "learned vs plain time at equal L1", "value": round(lr["speedup_at_equal_l1"], 4), "unit": "x",
             "higher_is_better": True, "n_gpus": 1, "dtype": "f32 (solver), bf16/f32 (network)",
@@ -146,6 +153,28 @@ def learned_line(seeds=(5000, 5001, 5002), coarse_steps: int = 200):
             "speedup_at_equal_l1_n3_fit": lr["speedup_at_equal_l1_fit"],
             "l1_gain_vs_plain_1024": lr["l1_gain_vs_plain_1024"], "speedup_vs_plain_2048": lr["speedup_vs_plain_2048"],
             "vs_baseline": None}
+
+
+def learned_subline(timeout_s: int = 240):
+    """The learned comparison for the headline line, on one unseen seed (about
+    a minute): run as a child process, because the solver library and torch
+    each bring a HIP runtime and a process can use only the one it loaded
+    first."""
+    cmd = [sys.executable, os.path.abspath(__file__), "--workload", "learned", "--learned-quick"]
+    try:
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout_s)
+    except subprocess.TimeoutExpired:
+        return {"error": f"timed out after {timeout_s} s"}
+    if r.returncode != 0:
+        return {"error": f"exit {r.returncode}: " + r.stderr.strip().splitlines()[-1][:300] if r.stderr.strip() else
+                f"exit {r.returncode}"}
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    keep = ("value", "unit", "learned_l1", "learned_seconds", "plain_resolutions", "plain_l1", "plain_seconds",
+            "plain_n_at_equal_l1", "plain_seconds_at_equal_l1", "l1_gain_vs_plain_1024", "speedup_vs_plain_2048")
+    out = {"metric": "learned vs plain time at equal L1 (speed-up)"}
+    out.update({k: d[k] for k in keep if k in d})
+    out["config"] = d.get("config")
+    return out
 
 
 def traffic_from_profiles(workload: str):
@@ -284,6 +313,10 @@ def main():
     ap.add_argument("--tile-rows", type=int, default=None, help="default: the autotuned work plan of the class")
     ap.add_argument("--graph", action="store_true", help="capture the timed steps in a HIP graph")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="leave the learned comparison out of the step-channel line")
+    ap.add_argument("--learned-quick", action="store_true",
+                    help="learned: one unseen seed and plain runs up to 1536^2 (the headline line's sub-object)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -295,7 +328,10 @@ def main():
         print(json.dumps(copy_bandwidth()), flush=True)
         return
     if args.workload == "learned":
-        print(json.dumps(learned_line()), flush=True)
+        if args.learned_quick:
+            print(json.dumps(learned_line(seeds=(5000,), resolutions=(1024, 1152, 1280, 1536))), flush=True)
+        else:
+            print(json.dumps(learned_line()), flush=True)
         return
     if args.workload == "unstructured":
         print(json.dumps(uns_line(args.steps, args.warmup, args.refine, not args.no_cpu_baseline)), flush=True)
@@ -340,8 +376,13 @@ def main():
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
     # counters were taken at the default sizes (1024 per block side, 8192 for muscl-quadrants)
     traffic = traffic_from_profiles(f"{args.workload}_{precision}") if args.n == 1024 else None
-    kernel = ("k_muscl (the step's update launch: main list and fix-up rounds)"
-              if cases[0].scheme == "MUSCLHancock" else "k_godunov")
+    if cases[0].scheme != "MUSCLHancock":
+        kernel = "k_godunov"
+    elif len(cases) == 1 and len(cases[0].blocks) > 1:
+        kernel = ("the step's MUSCL update launches k_muscl (main list), k_muscl_guess and k_muscl_check, "
+                  "timed from the first one's start to the last one's end")
+    else:
+        kernel = "k_muscl"
 
     line = {
         "metric": METRIC,
@@ -378,6 +419,8 @@ def main():
         line["batched_speedup_vs_looped"] = round(loop / wall, 2)
     if not args.no_cpu_baseline and args.workload == "step-channel":
         line["cpu_baseline"] = cpu_baseline()
+    if not args.no_learned and args.workload == "step-channel":
+        line["learned"] = learned_subline()
     print(json.dumps(line), flush=True)
 
 
