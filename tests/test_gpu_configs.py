@@ -73,11 +73,14 @@ def test_config4_full_workload_batched_matches_looped_and_oracle():
             assert l1 < 1e-5 and linf < 1e-3
 
 
-# Relative L1 / L-inf bounds of the FP32 path against FP64, per configuration.
-# Config 1 runs the unsplit first-order scheme at CFL 0.8, beyond its 2-D
-# stability limit (the Courant numbers of both directions sum to about 1.6):
-# rounding differences grow there, so its bound is loose.
-FP32_BOUNDS = {"C1": (0.1, 1.0), "C2": (1e-3, 0.1), "C3": (1e-3, 0.1), "C4": (1e-3, 0.1), "C5": (1e-3, 0.1)}
+# Relative L1 / L-inf bounds of the FP32 path against FP64, per configuration,
+# about 5-10x above what they measure (largest over the four conserved
+# variables: C1 4.2e-5 / 2.6e-2, C2 5.2e-5 / 4.0e-3, C3 2.4e-5 / 6.5e-4,
+# C4 1.7e-7 / 2.7e-5, C5 8.0e-5 / 1.4e-3).  Config 1 runs the unsplit
+# first-order scheme at CFL 0.8, beyond its 2-D stability limit (the Courant
+# numbers of both directions sum to about 1.6), where rounding differences
+# grow, so its bound keeps more room.
+FP32_BOUNDS = {"C1": (1e-3, 0.1), "C2": (5e-4, 0.02), "C3": (2e-4, 5e-3), "C4": (2e-6, 2e-4), "C5": (5e-4, 1e-2)}
 
 
 @pytest.mark.parametrize("name", ["C1", "C2", "C3", "C4", "C5"])
