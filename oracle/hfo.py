@@ -17,6 +17,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libhf_oracle.so")
 REF_HARNESS = os.path.join(HERE, "_ref", "ref_harness")
+REF_HARNESS_OMP = os.path.join(HERE, "_ref", "ref_harness_omp")   # OpenMP pragmas on: timing only
 
 _lib = None
 _dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
@@ -175,23 +176,32 @@ def have_reference() -> bool:
     return os.path.exists(REF_HARNESS)
 
 
-def run_reference(case, nsteps: int, fields: Optional[Sequence[np.ndarray]] = None, timeout: float = 3600):
+def run_reference(case, nsteps: int, fields: Optional[Sequence[np.ndarray]] = None, timeout: float = 3600,
+                  omp_threads: int = 0):
     """Run the reference's own classes (oracle/_ref/ref_harness) on a case.
-    Returns (fields per block, all TimeStep calls, step dts, elapsed seconds)."""
+    Returns (fields per block, all TimeStep calls, step dts, elapsed seconds).
+    With omp_threads the build with the reference's OpenMP pragmas enabled
+    runs on that many threads: for timing only (its TimeStep minimum is
+    updated without a reduction)."""
     if not have_reference():
         raise RuntimeError("reference harness not built")
+    harness, env = REF_HARNESS, None
+    if omp_threads:
+        if not os.path.exists(REF_HARNESS_OMP):
+            raise RuntimeError("OpenMP reference harness not built")
+        harness, env = REF_HARNESS_OMP, dict(os.environ, OMP_NUM_THREADS=str(omp_threads))
     with tempfile.TemporaryDirectory() as td:
         cj = os.path.join(td, "case.json")
         out = os.path.join(td, "out.bin")
         case.write_json(cj)
-        args = [REF_HARNESS, cj, str(nsteps), out]
+        args = [harness, cj, str(nsteps), out]
         if fields is not None:
             fin = os.path.join(td, "fields.bin")
             with open(fin, "wb") as f:
                 for U in fields:
                     f.write(np.ascontiguousarray(U, np.float64).tobytes())
             args.append(fin)
-        subprocess.run(args, check=True, stdout=subprocess.DEVNULL, timeout=timeout)
+        subprocess.run(args, check=True, stdout=subprocess.DEVNULL, timeout=timeout, env=env)
         return read_ref_output(out)
 
 
