@@ -50,9 +50,9 @@ __device__ __forceinline__ double shfl_dn1(double x)
 __device__ __forceinline__ float shfl_up1(float x) { return __int_as_float(dpp_up(__float_as_int(x))); }
 __device__ __forceinline__ float shfl_dn1(float x) { return __int_as_float(dpp_dn(__float_as_int(x))); }
 
-// Work items of a persistent launch: wave w first takes item w for w below
-// nstatic (no atomic, so a launch does not open with every wave hitting one
-// counter), then tickets from the queue counter, offset by nstatic.  Every
+// Work items of a persistent launch: wave w first takes item w (no atomic, so
+// a launch does not open with every wave hitting one counter), then tickets
+// from the queue counter, offset by the number of waves.  Every
 // lane executes the atomic (only lane 0 adds 1) so that no lane-divergent
 // region surrounds the loop's exit test; lane 0's old value is the wave's
 // ticket.  A lane-0-only branch here let the compiler's control-flow
@@ -69,13 +69,20 @@ __device__ __forceinline__ unsigned next_ticket(uint32_t* sync, int lane, unsign
     const unsigned t = atomicAdd(&sync[HF_SYNC_QUEUE], lane == 0 ? 1u : 0u);
     return __builtin_amdgcn_readfirstlane(t) + base;
 }
-// nstatic items are handed out without the counter; counter value c is then
-// item base + c (base = nstatic, or further on when the launch starts later
-// in the list)
-__device__ __forceinline__ unsigned first_ticket(uint32_t* sync, int lane, unsigned nstatic, unsigned base)
+// Tickets of a launch over items [0, n): wave w takes item w, then counter
+// tickets.  When the items are no more than the waves, every wave takes at
+// most its own item and no wave touches the counter: a wave would only learn
+// there that the list is done, and those atomics all arrive together when the
+// waves finish together and serialise at the counter.
+__device__ __forceinline__ unsigned ticket_first(unsigned n)
 {
     const unsigned w = wave_id();
-    return w < nstatic ? w : next_ticket(sync, lane, base);
+    return (n <= total_waves() && w >= n) ? n : w;   // otherwise w < waves < n
+}
+__device__ __forceinline__ unsigned ticket_next(uint32_t* sync, int lane, unsigned n)
+{
+    const unsigned nw = total_waves();
+    return n <= nw ? n : next_ticket(sync, lane, nw);
 }
 
 // What waves of one launch tell each other travels only through device-scope
@@ -541,9 +548,7 @@ __global__ __launch_bounds__(256) void k_muscl(const HfStepArgs<typename M::R> a
     const unsigned nsk = (a.nskip && !a.use_fixed) ? __builtin_amdgcn_readfirstlane(*a.nskip) : 0u;
     if (nsk == 0u) {   // every item of the list
         const unsigned n = (unsigned)a.nmain;
-        const unsigned nstatic = min(total_waves(), n);
-        for (unsigned idx = first_ticket(a.sync, lane, nstatic, nstatic); idx < n;
-             idx = next_ticket(a.sync, lane, nstatic))
+        for (unsigned idx = ticket_first(n); idx < n; idx = ticket_next(a.sync, lane, n))
             main_item<M, LIM, RESID>(a, idx, lane, cached, c_step, c_pred, c_kstep, c_kpred);
         return;
     }
@@ -557,11 +562,10 @@ __global__ __launch_bounds__(256) void k_muscl(const HfStepArgs<typename M::R> a
         lane_prefix(cnt, tot);
         n += tot;
     }
-    const unsigned nstatic = min(total_waves(), n);
     RangeCursor cur;
     main_chunk<typename M::R>(a, cur, 0);
     cur.base = 0u;
-    for (unsigned t = first_ticket(a.sync, lane, nstatic, nstatic); t < n; t = next_ticket(a.sync, lane, nstatic)) {
+    for (unsigned t = ticket_first(n); t < n; t = ticket_next(a.sync, lane, n)) {
         while (t >= cur.base + cur.total) {
             const unsigned nb_ = cur.base + cur.total;
             if (cur.c0 + 64 >= nseg) return;   // not reached: the chunks hold n items
@@ -645,13 +649,12 @@ __global__ __launch_bounds__(256) void k_muscl_guess(const HfStepArgs<typename M
             n += ck.total;
         }
     if (n == 0u) return;
-    const unsigned nstatic = min(total_waves(), n);
     g = g0;
     cur.base = 0u;
     int cb = -1;   // the block whose dts this wave holds
     double dt_step = 0.0, dt_pred = 0.0;
     long long kpred = 0;
-    for (unsigned t = first_ticket(a.sync, lane, nstatic, nstatic); t < n; t = next_ticket(a.sync, lane, nstatic)) {
+    for (unsigned t = ticket_first(n); t < n; t = ticket_next(a.sync, lane, n)) {
         while (t >= cur.base + cur.total) {
             const unsigned nb_ = cur.base + cur.total;
             int c0 = cur.c0 + 64;
@@ -852,9 +855,8 @@ __global__ __launch_bounds__(256) void k_godunov(const HfStepArgs<typename M::R>
 {
     typedef typename M::R R;
     const int lane = threadIdx.x & 63;
-    const unsigned nstatic = min(total_waves(), (unsigned)a.nitems);
-    for (unsigned idx = first_ticket(a.sync, lane, nstatic, nstatic);; idx = next_ticket(a.sync, lane, nstatic)) {
-        if (idx >= (unsigned)a.nitems) break;
+    const unsigned n = (unsigned)a.nitems;
+    for (unsigned idx = ticket_first(n); idx < n; idx = ticket_next(a.sync, lane, n)) {
         const HfItem it = a.items[a.item0 + idx];
         double dt_step, dt_pred;
         if (a.use_fixed) {
