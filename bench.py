@@ -217,6 +217,13 @@ def uns_line(steps: int, warmup: int, refine: int, with_cpu: bool):
     s.synchronize()
     wall = time.perf_counter() - t0
     launches, kernel_ms = s.kernel_stats()
+    s.profile(False)
+    s.run(steps, graph=True)       # capture, then time the same loop as a HIP graph
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.run(steps, graph=True)
+    s.synchronize()
+    wall_graph = time.perf_counter() - t0
     s.close()
     n = len(q)
     avg_ms = kernel_ms / launches
@@ -233,6 +240,9 @@ def uns_line(steps: int, warmup: int, refine: int, with_cpu: bool):
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(f"unstructured_r{refine}"),
                      "kernel": "k_uns_step", "avg_launch_ms": round(avg_ms, 5), "launches": launches,
                      "algorithmic_bytes_per_launch": n * bpc, "bytes_per_cell": bpc},
+        "hip_graph": {"ms_per_step": round(wall_graph / steps * 1e3, 5),
+                      "value": round(n * steps / wall_graph / 1e6, 2),
+                      "note": "the same steps captured in a HIP graph (one replay, after the capturing run)"},
     }
     if with_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

@@ -9,6 +9,8 @@
 #include <memory>
 #include <string>
 #include <unordered_map>
+#include <map>
+#include <tuple>
 #include <vector>
 
 #include "../../include/hyperflow_uns.h"
@@ -33,6 +35,13 @@ struct hf_uns {
     int hist_cap = 1 << 20;
     int64_t calls = 0;
     hfu::UnsMesh mesh;
+    // graph-captured loops: the device's TimeStep call counter (two words,
+    // alternating by parity), its pinned staging word, and the graphs by
+    // (steps, slot, buffer parity, call-count parity)
+    unsigned long long* d_kcall = nullptr;
+    unsigned long long* h_kcall = nullptr;
+    hipEvent_t kcall_copied = nullptr;
+    std::map<std::tuple<int, int, int, int>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
     bool profile = false;                    // HIP events around every step launch
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending, ev_free;
     int64_t prof_launches = 0;
@@ -256,6 +265,9 @@ int hf_uns_create(const hf_uns_desc* d, hf_uns** out)
     HF_HIPU(hipMalloc(&U->d_slots, sizeof(unsigned long long) * 3 * hfu::SLOT_WORDS));
     HF_HIPU(hipMemset(U->d_slots, 0, sizeof(unsigned long long) * 3 * hfu::SLOT_WORDS));
     HF_HIPU(hipMalloc(&U->d_hist, sizeof(double) * U->hist_cap));
+    HF_HIPU(hipMalloc(&U->d_kcall, sizeof(unsigned long long) * 2));
+    HF_HIPU(hipHostMalloc((void**)&U->h_kcall, sizeof(unsigned long long), hipHostMallocDefault));
+    HF_HIPU(hipEventCreateWithFlags(&U->kcall_copied, hipEventDisableTiming));
     HF_HIPU(hipMemcpy(U->d_nbr, nbr_em.data(), sizeof(int32_t) * 4 * n, hipMemcpyHostToDevice));
     HF_HIPU(hipMemcpy(U->d_len, len.data(), sizeof(double) * 4 * n, hipMemcpyHostToDevice));
     HF_HIPU(hipMemcpy(U->d_cs, cst.data(), sizeof(double) * cst.size(), hipMemcpyHostToDevice));
@@ -287,13 +299,19 @@ void hf_uns_destroy(hf_uns* u)
         return;
     }
     if (u->stream) (void)hipStreamSynchronize(u->stream);
+    for (auto& kv : u->graphs) {
+        (void)hipGraphExecDestroy(kv.second.second);
+        (void)hipGraphDestroy(kv.second.first);
+    }
     for (auto* list : {&u->ev_pending, &u->ev_free})
         for (auto& e : *list) {
             (void)hipEventDestroy(e.first);
             (void)hipEventDestroy(e.second);
         }
+    if (u->kcall_copied) (void)hipEventDestroy(u->kcall_copied);
+    if (u->h_kcall) (void)hipHostFree(u->h_kcall);
     for (void* p : {(void*)u->d_nbr, (void*)u->d_len, (void*)u->d_cs, (void*)u->d_iv, (void*)u->buf[0],
-                    (void*)u->buf[1], (void*)u->d_slots, (void*)u->d_hist})
+                    (void*)u->buf[1], (void*)u->d_slots, (void*)u->d_hist, (void*)u->d_kcall})
         if (p) (void)hipFree(p);
     if (u->stream) (void)hipStreamDestroy(u->stream);
     delete u;
@@ -379,7 +397,7 @@ int hf_uns_advance(hf_uns* u, double dt)
 int hf_uns_run(hf_uns* u, int nsteps, int flags)
 {
     if (!u) return fail(HF_EINVAL, "hf_uns_run: null context");
-    if (flags != 0) return fail(HF_EINVAL, "hf_uns_run: flags must be 0");
+    if (flags & ~HF_UNS_RUN_GRAPH) return fail(HF_EINVAL, "hf_uns_run: unknown flags");
     if (nsteps < 0) return fail(HF_EINVAL, "hf_uns_run: nsteps must be >= 0");
     if (nsteps == 0) return HF_OK;
     int rc = ensure_slot(u);
@@ -389,6 +407,51 @@ int hf_uns_run(hf_uns* u, int nsteps, int flags)
     static const std::vector<unsigned long long> empty(hfu::SLOT_WORDS, hfu::SLOT_EMPTY);
     HF_HIP(hipMemcpyAsync(u->d_slots + (size_t)((u->slot + 1) % 3) * hfu::SLOT_WORDS, empty.data(),
                           sizeof(unsigned long long) * empty.size(), hipMemcpyHostToDevice, u->stream));
+    if (flags & HF_UNS_RUN_GRAPH) {
+        // the call counter goes to the device (word calls % 2), outside the
+        // graph; every launch argument of the loop then depends only on the
+        // slot, the buffer parity and the call-count parity at its start
+        const int par = (int)(u->calls & 1);
+        HF_HIP(hipEventSynchronize(u->kcall_copied));   // the previous copy has read the staging word
+        *u->h_kcall = (unsigned long long)u->calls;
+        HF_HIP(hipMemcpyAsync(u->d_kcall + par, u->h_kcall, sizeof(unsigned long long), hipMemcpyHostToDevice,
+                              u->stream));
+        HF_HIP(hipEventRecord(u->kcall_copied, u->stream));
+        const auto key = std::make_tuple(nsteps, u->slot, u->cur, par);
+        auto it = u->graphs.find(key);
+        if (it == u->graphs.end()) {
+            hipGraph_t g = nullptr;
+            hipGraphExec_t ge = nullptr;
+            HF_HIP(hipStreamBeginCapture(u->stream, hipStreamCaptureModeThreadLocal));
+            int cur = u->cur, slot = u->slot;
+            hipError_t le = hipSuccess;
+            for (int i = 0; i < nsteps && le == hipSuccess; i++) {
+                hfu::UnsStep s = step_args(u);
+                s.A = u->buf[cur];
+                s.B = u->buf[cur ^ 1];
+                s.rd = slot;
+                s.wr = (slot + 1) % 3;
+                s.rs = (slot + 2) % 3;
+                s.kdev = u->d_kcall;
+                s.kpar = (par + i) & 1;
+                le = hfu::launch_uns_step(u->mesh, s, u->stream);
+                cur ^= 1;
+                slot = s.wr;
+            }
+            const hipError_t ce = hipStreamEndCapture(u->stream, &g);
+            if (le != hipSuccess || ce != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                HF_HIP(le != hipSuccess ? le : ce);
+            }
+            HF_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+            it = u->graphs.emplace(key, std::make_pair(g, ge)).first;
+        }
+        HF_HIP(hipGraphLaunch(it->second.second, u->stream));
+        u->calls += nsteps;
+        u->cur ^= (nsteps & 1);
+        u->slot = (u->slot + nsteps) % 3;
+        return HF_OK;
+    }
     for (int i = 0; i < nsteps; i++) {
         hfu::UnsStep s = step_args(u);
         s.rd = u->slot;

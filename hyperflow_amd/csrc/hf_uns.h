@@ -40,9 +40,13 @@ struct UnsStep {
     unsigned long long* slots;   // [3][SLOT_WORDS]
     int32_t rd, wr, rs, use_fixed;
     double fixed_dt;
-    long long kcall;         // TimeStep call index (1-based) of this step
+    long long kcall;         // TimeStep call index (1-based) of this step, unless kdev is set
     double* hist;
-    int32_t hist_cap, pad;
+    int32_t hist_cap;
+    // graph-captured steps: the call index is kdev[kpar] + 1, and workgroup 0
+    // stores it in kdev[kpar ^ 1] for the next step
+    int32_t kpar;
+    unsigned long long* kdev;
 };
 
 // e0/e1 (optional): events stamped at the dispatch's start and end

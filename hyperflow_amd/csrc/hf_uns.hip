@@ -86,14 +86,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const double gamma = m.gamma, g1 = m.gamma - 1.0;
     const int lane = threadIdx.x & 63;
     double dt;
+    const long long kcall = s.kdev ? (long long)s.kdev[s.kpar] + 1 : s.kcall;
     if (s.use_fixed) {
         dt = s.fixed_dt;
     } else {
-        const double cf = (s.kcall <= 10) ? 0.2 * m.cfl : m.cfl;   // timestep.cpp:57-61
+        const double cf = (kcall <= 10) ? 0.2 * m.cfl : m.cfl;   // timestep.cpp:57-61
         dt = cf * double_of(wave_min_bits(s.slots[s.rd * SLOT_WORDS + lane * SLOT_STRIDE]));
     }
     if (blockIdx.x == 0 && threadIdx.x < 64) {
-        if (threadIdx.x == 0 && !s.use_fixed && s.kcall >= 1 && s.kcall <= s.hist_cap) s.hist[s.kcall - 1] = dt;
+        if (threadIdx.x == 0 && s.kdev) s.kdev[s.kpar ^ 1] = (unsigned long long)kcall;
+        if (threadIdx.x == 0 && !s.use_fixed && kcall >= 1 && kcall <= s.hist_cap) s.hist[kcall - 1] = dt;
         if (s.rs >= 0) s.slots[s.rs * SLOT_WORDS + lane * SLOT_STRIDE] = SLOT_EMPTY;
     }
     unsigned long long cand = ~0ull;

@@ -355,8 +355,10 @@ class UnsSolver:
     def advance(self, dt: float) -> None:
         _check(_lib().hf_uns_advance(self._h, float(dt)), "hf_uns_advance")
 
-    def run(self, nsteps: int) -> None:
-        _check(_lib().hf_uns_run(self._h, int(nsteps), 0), "hf_uns_run")
+    def run(self, nsteps: int, graph: bool = False) -> None:
+        """nsteps device-resident steps; graph=True captures them in a HIP
+        graph (cached per step count and starting state)."""
+        _check(_lib().hf_uns_run(self._h, int(nsteps), 1 if graph else 0), "hf_uns_run")
 
     def synchronize(self) -> None:
         _check(_lib().hf_uns_synchronize(self._h), "hf_uns_synchronize")

@@ -59,7 +59,10 @@ HF_API int hf_uns_time_step(hf_uns* u, double* dt);
 HF_API int hf_uns_advance(hf_uns* u, double dt);
 /* nsteps of Simulation::solve's step body (simulation.cpp:85-118, without
  * clamps or output) with the dt kept on the device; asynchronous.  One launch
- * per step; flags must be 0 (no graph mode: a step is a single launch). */
+ * per step; flags: 0, or HF_UNS_RUN_GRAPH to capture the loop in a HIP graph
+ * (cached per step count and starting state, replayed on later calls; the
+ * TimeStep call counter then lives on the device). */
+enum { HF_UNS_RUN_GRAPH = 1 };
 HF_API int hf_uns_run(hf_uns* u, int nsteps, int flags);
 HF_API int hf_uns_synchronize(hf_uns* u);
 /* Every TimeStep result so far, in call order. */

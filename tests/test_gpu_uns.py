@@ -76,6 +76,34 @@ def test_step_by_step_matches_fused_run(path):
     s.close()
 
 
+@pytest.mark.parametrize("path", [p for p in GOLDEN if "nonphysical" not in p][:1], ids=lambda p: os.path.basename(p)[:-4])
+def test_graph_replays_match_eager(path):
+    """HIP-graph loops (cached per step count and starting state; the call
+    counter on the device) equal eager steps bit for bit, over several
+    replays of the same graphs with host work in between, odd counts that
+    change the buffer, slot and call-count parities, the first ten calls'
+    reduced CFL inside a graph, and a step-by-step interlude."""
+    g, cj, inlet, gamma, cfl = load(path)
+    say(f"uns graph vs eager {os.path.basename(path)}")
+    plan = [3, 7, 3, 7, 6, 5, 3, 7]
+    eager = U.UnsSolver(g["quads"], g["codes"], inlet, gamma, cfl)
+    graph = U.UnsSolver(g["quads"], g["codes"], inlet, gamma, cfl)
+    for s in (eager, graph):
+        s.upload(g["init"])
+    for k, n in enumerate(plan):
+        eager.run(n)
+        graph.run(n, graph=True)
+        graph.synchronize()
+        np.testing.assert_array_equal(graph.download(), eager.download())   # host work between replays
+        if k == 4:   # a host-driven step in between
+            for s in (eager, graph):
+                s.advance(s.time_step())
+    np.testing.assert_array_equal(graph.dt_history(), eager.dt_history())
+    np.testing.assert_array_equal(graph.download(), eager.download())
+    eager.close()
+    graph.close()
+
+
 def test_random_mesh_and_states_match_oracle():
     rng = np.random.default_rng(11)
     q, c = U.box(37, 29, code=U.TRANSMISSIVE, skew=0.35)
@@ -145,7 +173,7 @@ def test_bad_sizes_are_rejected():
     with pytest.raises(Exception, match="nsteps"):
         U._check(U._lib().hf_uns_run(s._h, -1, 0), "hf_uns_run")
     with pytest.raises(Exception, match="flags"):
-        U._check(U._lib().hf_uns_run(s._h, 1, 1), "hf_uns_run")
+        U._check(U._lib().hf_uns_run(s._h, 1, 2), "hf_uns_run")   # 1 = HF_UNS_RUN_GRAPH; 2 is unknown
     s.close()
 
 
