@@ -20,6 +20,8 @@ corrected solver actually visits).
 
 Training runs on the GPU with the library's own kernels (hf_net_train_step:
 BF16 MFMA forward and backward, Adam).  torch only provides device memory.
+A second stage (learned_unroll.py) trains the network through the coarse
+solver itself, with the rollout's own boundaries and fallbacks.
 
     python -m hyperflow_amd.learned train [--seeds 40] [--every 4] [--rounds 2] [--steps 30000]
     python -m hyperflow_amd.learned evaluate [--weights hyperflow_amd/learned_weights.npz]

@@ -207,8 +207,8 @@ def test_trained_network_gain_over_plain_step():
     """The committed network against the plain coarse step.  One correction
     (K coarse steps from the restricted fine state of held-out seeds) must
     remove a clear part of the density error; a 200-step rollout on unseen
-    seeds, corrected every 8 steps from step 24, must end closer to the
-    restricted 2048^2 solution than the plain 1024^2 run."""
+    seeds, corrected at the stored interval from the stored first step, must
+    end closer to the restricted 2048^2 solution than the plain 1024^2 run."""
     from hyperflow_amd import learned as L
     net = L.load()
     meta = L.load_meta()
