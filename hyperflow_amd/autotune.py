@@ -207,7 +207,10 @@ def _workloads(names):
     for name in names:
         if name == "godunov-quadrants":
             c = C.godunov_quadrants(0)
-            out.append(("godunov-quadrants", [c], [c.initial_fields()[0]], "fp64", "compat", False, {}))
+            # 160k cells fill a fraction of the resident waves: short items
+            # keep more SIMDs busy (a Godunov item has one halo row per side)
+            out.append(("godunov-quadrants", [c], [c.initial_fields()[0]], "fp64", "compat", False,
+                        {"rows": (2, 3, 4, 6, 8, 12, 16), "tails": (1, 2, 3, 4, 6, 8)}))
         elif name == "step-channel":
             c = C.muscl_step_channel(1024)
             # past the first 10 TimeStep calls' reduced CFL, as the benchmark times it

@@ -137,15 +137,28 @@ __device__ __forceinline__ unsigned long long uniform_bits(double x)
     return ((unsigned long long)hi << 32) | lo;
 }
 
-// Per-wave maxima into the block's slots; every lane holds the reduced value,
-// so every lane may issue the (idempotent) atomic -- no divergent region.  The
-// value is passed as a scalar: the compiler then issues one atomic from one
-// lane, instead of first reducing 64 lane values it cannot tell are equal (a
-// 64-iteration scalar loop per atomic).
+// Raises the slot at p to the wave-uniform bits b.  A slot is cleared before
+// the launch that fills it and only grows during it, so a wave whose value
+// does not exceed the slot's current one (a coherent load) leaves it alone:
+// the same final maximum, and the device-scope atomics on one address, which
+// serialise, come only from the waves that raise it (on a 400^2 block about
+// 460 items end together and mostly find the maximum already there).
+__device__ __forceinline__ void raise_max(unsigned long long* p, unsigned long long b)
+{
+    const unsigned long long c = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)c);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(c >> 32));
+    if ((((unsigned long long)hi << 32) | lo) < b) atomicMax(p, b);
+}
+
+// Per-wave maxima into the block's slots; every lane holds the reduced value.
+// The value is passed as a scalar: the compiler then issues one atomic from
+// one lane, instead of first reducing 64 lane values it cannot tell are equal
+// (a 64-iteration scalar loop per atomic).
 __device__ __forceinline__ void store_maxima(unsigned long long* out, int block, double mx, double my)
 {
-    atomicMax(&out[2 * block], uniform_bits(mx));
-    atomicMax(&out[2 * block + 1], uniform_bits(my));
+    raise_max(&out[2 * block], uniform_bits(mx));
+    raise_max(&out[2 * block + 1], uniform_bits(my));
 }
 
 // Load row y of this lane's column: ghost rows/columns come from the ghost
