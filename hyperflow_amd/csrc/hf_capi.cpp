@@ -276,6 +276,7 @@ HfStepArgs<R> step_args(hf_ctx* c, int item0, int nitems, const void* A, const v
     a.fixrange = c->d_rounds + fixrange_offset(*c);
     a.nskip = nullptr;
     a.wait_ticks = c->wait_ticks;
+    a.probe_max = c->probe_max;
     a.nrounds = 0;
     a.pending = c->d_pending;
     a.npending = c->npending;
@@ -712,6 +713,12 @@ int build_plan(hf_ctx* c, const hf_case_desc* desc, bool dense)
     }
     c->main_begin = perblock_end;
     c->main_items = (int)c->items.size() - perblock_end;
+    // Wave maxima read their slot before raising it where hundreds of a
+    // block's items end together (raise_max in hf_kernels.h); not for
+    // Godunov, whose items are short, nor for blocks of few items.
+    int most = 0;
+    for (int b = 0; b < c->nb; b++) most = std::max(most, c->item_count[b]);
+    c->probe_max = (c->scheme == HF_MUSCL_HANCOCK && most >= 256) ? 1 : 0;
     c->fixrange.assign(2 * (size_t)c->nb, 0);
     if (c->scheme == HF_MUSCL_HANCOCK && c->dt_mode == HF_DT_COMPAT)
         for (int lb = 1; lb < c->maxnb; lb++) {

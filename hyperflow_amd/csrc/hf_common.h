@@ -73,6 +73,7 @@ struct HfStepArgs {
     const int32_t* fixrange;
     const uint32_t* nskip;
     unsigned long long wait_ticks;       // bound of an in-launch wait in s_memrealtime ticks (10 ms)
+    int32_t probe_max;                   // 1: a wave reads a maxima slot before raising it (raise_max)
     uint32_t* pending;                   // [npending] bitmap over the fix-up items left to the check launch's last wave
     // Fix-up state of the step.  Slot 0 belongs to the guess launch, slot 1 to
     // the check launch: imax_fix[s * 2 * nblocks + 2 * b + axis] holds the

@@ -44,6 +44,7 @@ struct hf_ctx {
     int maxnb = 1;
     int wg_cap = 1024;                            // resident workgroups of an update launch
     unsigned long long wait_ticks = 1000000;      // bound of an in-launch wait (s_memrealtime ticks)
+    int probe_max = 0;                            // HfStepArgs::probe_max of the context's launches
     int64_t total = 0;                            // elements per state buffer
     void* buf[2] = {nullptr, nullptr};
     int cur = 0;

@@ -8,7 +8,8 @@
 // from HBM (plus the 2-lane halo at each strip edge and 2 halo rows per
 // strip) and every output written once; nothing intermediate goes to memory.
 // The new state's max wave speeds (for the next dt) are reduced in the same
-// pass: wave shuffles, then one 64-bit atomic max per wave.
+// pass: wave shuffles, then a 64-bit atomic max per wave whose value raises
+// the block's slot (raise_max).
 //
 // The interior is read from A and the filled ghosts from G (the ghost fill
 // writes them into the destination buffer), so no launch ever reads a cell
@@ -87,7 +88,8 @@ __device__ __forceinline__ unsigned ticket_next(uint32_t* sync, int lane, unsign
 
 // What waves of one launch tell each other travels only through device-scope
 // atomics (wave-speed maxima, fixed flags, the pending bitmap, the counters):
-// writers update with atomic read-modify-writes, and readers of values that
+// writers update with atomic read-modify-writes (a maxima writer skips one
+// that a coherent load shows cannot raise the slot), and readers of values that
 // decide results read with coherent_read (hf_physics.h), an atomic
 // read-modify-write that changes nothing.  The new states themselves are not
 // read inside the launch.
@@ -141,8 +143,11 @@ __device__ __forceinline__ unsigned long long uniform_bits(double x)
 // the launch that fills it and only grows during it, so a wave whose value
 // does not exceed the slot's current one (a coherent load) leaves it alone:
 // the same final maximum, and the device-scope atomics on one address, which
-// serialise, come only from the waves that raise it (on a 400^2 block about
-// 460 items end together and mostly find the maximum already there).
+// serialise, come only from the waves that raise it.  The load puts a memory
+// round trip at the end of the item: it pays where hundreds of items of one
+// block end together (the 8-block channel: 4 % per step) and costs where few
+// do (a 400^2 Godunov block: 15 %; 512 small blocks: 1.5 %), so the host
+// enables it per context (HfStepArgs::probe_max).
 __device__ __forceinline__ void raise_max(unsigned long long* p, unsigned long long b)
 {
     const unsigned long long c = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -155,10 +160,16 @@ __device__ __forceinline__ void raise_max(unsigned long long* p, unsigned long l
 // The value is passed as a scalar: the compiler then issues one atomic from
 // one lane, instead of first reducing 64 lane values it cannot tell are equal
 // (a 64-iteration scalar loop per atomic).
-__device__ __forceinline__ void store_maxima(unsigned long long* out, int block, double mx, double my)
+__device__ __forceinline__ void store_maxima(unsigned long long* out, int block, double mx, double my, bool probe)
 {
-    raise_max(&out[2 * block], uniform_bits(mx));
-    raise_max(&out[2 * block + 1], uniform_bits(my));
+    const unsigned long long bx = uniform_bits(mx), by = uniform_bits(my);
+    if (probe) {
+        raise_max(&out[2 * block], bx);
+        raise_max(&out[2 * block + 1], by);
+    } else {
+        atomicMax(&out[2 * block], bx);
+        atomicMax(&out[2 * block + 1], by);
+    }
 }
 
 // Load row y of this lane's column: ghost rows/columns come from the ghost
@@ -267,7 +278,7 @@ __device__ __forceinline__ void muscl_item(const HfStepArgs<typename M::R>& a, c
         }
         for (int k = 0; k < 4; k++) { Gs[k] = Gn[k]; Tc[k] = Tn[k]; uc[k] = up[k]; up[k] = un[k]; un[k] = nn[k]; }
     }
-    if (!RESID) store_maxima(imax_out, it.block, wave_max(mx), wave_max(my));
+    if (!RESID) store_maxima(imax_out, it.block, wave_max(mx), wave_max(my), a.probe_max != 0);
 #undef ROW
 }
 
@@ -374,7 +385,7 @@ __device__ __forceinline__ void muscl_item_t(const HfStepArgs<typename M::R>& a,
         }
         for (int k = 0; k < 4; k++) { Fw[k] = Fe[k]; Rc[k] = Rn[k]; uc[k] = up[k]; up[k] = un[k]; un[k] = nn[k]; }
     }
-    if (!RESID) store_maxima(imax_out, it.block, wave_max(mx), wave_max(my));
+    if (!RESID) store_maxima(imax_out, it.block, wave_max(mx), wave_max(my), a.probe_max != 0);
 #undef COL
 }
 
@@ -858,7 +869,7 @@ __device__ __forceinline__ void godunov_item(const HfStepArgs<typename M::R>& a,
         for (int k = 0; k < 4; k++) { Gs[k] = Gn[k]; un[k] = nn[k]; }
         sc = sn;
     }
-    if (!RESID) store_maxima(imax_out, it.block, wave_max(mx), wave_max(my));
+    if (!RESID) store_maxima(imax_out, it.block, wave_max(mx), wave_max(my), false);
 #undef ROW
 }
 
@@ -970,8 +981,9 @@ __global__ __launch_bounds__(256) void k_ghost(const HfGhost* __restrict__ tab, 
     // reduces to one value, and thread 0 merges consecutive waves of the same
     // block, so a workgroup issues one atomic per block it covers (device-
     // scope atomics on one address serialise: one per wave made this launch
-    // take 20 us on 66 k ghost cells, one per workgroup 13 us).  A wave
-    // spanning two blocks uses per-lane atomics.
+    // take 20 us on 66 k ghost cells, one per workgroup 13 us; reading the
+    // slot first, as raise_max does, made it slower still).  A wave spanning
+    // two blocks uses per-lane atomics.
     __shared__ int wblk[GHOST_WAVES];
     __shared__ double wmx[GHOST_WAVES], wmy[GHOST_WAVES];
     const int w = threadIdx.x >> 6;
