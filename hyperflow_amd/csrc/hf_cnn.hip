@@ -41,7 +41,7 @@ __device__ __forceinline__ bf16x8 zero8()
 // K-chunk c = 4s+g of MFMA step s is 8 input channels of one tap:
 // CIN = 32: tap s, channels 8g..8g+7;  CIN = 8: tap c (c >= 9 is zero).
 template <int CIN, int COUT, int MODE>
-__global__ __launch_bounds__(256) void k_conv(const ConvArgs a)
+__global__ __launch_bounds__(256, 3) void k_conv(const ConvArgs a)
 {
     constexpr int MT = (COUT + 15) / 16;
     constexpr int S = (9 * CIN / 8 + 3) / 4;
@@ -71,51 +71,10 @@ __global__ __launch_bounds__(256) void k_conv(const ConvArgs a)
         }
 
     uint32_t nfb = 0;
-    // CIN = 32: the B operand of tap (dy, dx) at row y is input row y+dy at
-    // column x+dx, so a window of three rows x three column shifts slides down
-    // with y and each new row costs three loads, issued one row ahead.
-    bf16x8 Bw[3][3], Bn[3];
-    if (CIN == 32) {
-#pragma unroll
-        for (int r = 0; r < 3; r++)
-#pragma unroll
-            for (int d = 0; d < 3; d++)
-                Bw[r][d] = load8(a.in + ((size_t)(y0 + r) * W2 + (x + d)) * CIN + g * 8);
-    }
-    for (int y = y0; y < y1; y++) {
-        f32x4 acc[MT];
-#pragma unroll
-        for (int mt = 0; mt < MT; mt++) acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        if (CIN == 32) {
-            const int yn = min(y + 3, a.ny + 1);   // padded row of input row y+2
-#pragma unroll
-            for (int d = 0; d < 3; d++) Bn[d] = load8(a.in + ((size_t)yn * W2 + (x + d)) * CIN + g * 8);
-#pragma unroll
-            for (int t = 0; t < 9; t++)
-#pragma unroll
-                for (int mt = 0; mt < MT; mt++)
-                    acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wA[mt][t], Bw[t / 3][t % 3], acc[mt], 0, 0, 0);
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                Bw[0][d] = Bw[1][d];
-                Bw[1][d] = Bw[2][d];
-                Bw[2][d] = Bn[d];
-            }
-        } else {
-#pragma unroll
-            for (int s = 0; s < S; s++) {
-                int tap = 4 * s + g;
-                const bool valid = tap < 9;
-                tap = valid ? tap : 8;
-                const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-                bf16x8 B = load8(a.in + ((size_t)(y + 1 + dy) * W2 + (x + 1 + dx)) * CIN);
-                B = valid ? B : zero8();
-#pragma unroll
-                for (int mt = 0; mt < MT; mt++)
-                    acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wA[mt][s], B, acc[mt], 0, 0, 0);
-            }
-        }
-
+    // A row's epilogue: bias, then ReLU (hidden layer), the stored
+    // activation's ReLU mask (data gradient), the FP32 output, or the
+    // correction applied to the solver state with its fallbacks.
+    auto finish = [&](const int y, const f32x4 (&acc)[MT]) {
         const size_t pix = (size_t)(y + 1) * W2 + (x + 1);
         if (MODE == CONV_HIDDEN || MODE == CONV_BWD) {
 #pragma unroll
@@ -170,6 +129,64 @@ __global__ __launch_bounds__(256) void k_conv(const ConvArgs a)
                     nfb++;
                 }
             }
+        }
+    };
+
+    if (CIN == 32) {
+        // The B operand of tap (dy, dx) at output row y is padded input row
+        // y+dy+1 at column x+dx: each output row needs three input rows of
+        // three column shifts, and each new row costs three loads.  Input
+        // rows live in a ring of RING buffers named at compile time (the row
+        // loop is unrolled by RING, so the window slides by renaming, not by
+        // copying registers, which would wait for the loads): the loads of
+        // row y+2+PF are issued while row y is computed, PF rows ahead of
+        // their use, so the memory latency overlaps PF rows of MFMAs.
+        constexpr int PF = 2, RING = 3 + PF;
+        bf16x8 Rb[RING][3];
+        auto load_row = [&](const int k, bf16x8 (&dst)[3]) {   // padded row y0 + k
+            const int yr = min(y0 + k, a.ny + 1);
+#pragma unroll
+            for (int d = 0; d < 3; d++) dst[d] = load8(a.in + ((size_t)yr * W2 + (x + d)) * CIN + g * 8);
+        };
+#pragma unroll
+        for (int k = 0; k < RING - 1; k++) load_row(k, Rb[k]);
+        for (int base = y0; base < y1; base += RING) {
+#pragma unroll
+            for (int u = 0; u < RING; u++) {
+                const int y = base + u;
+                if (y < y1) {
+                    load_row(y - y0 + RING - 1, Rb[(u + RING - 1) % RING]);
+                    f32x4 acc[MT];
+#pragma unroll
+                    for (int mt = 0; mt < MT; mt++) acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                    for (int t = 0; t < 9; t++)
+#pragma unroll
+                        for (int mt = 0; mt < MT; mt++)
+                            acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wA[mt][t], Rb[(u + t / 3) % RING][t % 3],
+                                                                             acc[mt], 0, 0, 0);
+                    finish(y, acc);
+                }
+            }
+        }
+    } else {
+        for (int y = y0; y < y1; y++) {
+            f32x4 acc[MT];
+#pragma unroll
+            for (int mt = 0; mt < MT; mt++) acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int s = 0; s < S; s++) {
+                int tap = 4 * s + g;
+                const bool valid = tap < 9;
+                tap = valid ? tap : 8;
+                const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+                bf16x8 B = load8(a.in + ((size_t)(y + 1 + dy) * W2 + (x + 1 + dx)) * CIN);
+                B = valid ? B : zero8();
+#pragma unroll
+                for (int mt = 0; mt < MT; mt++)
+                    acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wA[mt][s], B, acc[mt], 0, 0, 0);
+            }
+            finish(y, acc);
         }
     }
     if (MODE == CONV_APPLY) {
