@@ -802,9 +802,12 @@ __global__ __launch_bounds__(256) void k_muscl_check(const HfStepArgs<typename M
 // GodunovScheme::operator() (scheme/godunov/godunov.cpp:27-118) fused with
 // the forward Euler update.  Each cell's primitives and sound speed are
 // computed once and shared by its four faces.  Output columns per wave: 62.
-template <class M, bool RESID>
+// dt_of() gives the step's dt; it is called after the item's first rows are
+// requested and the first face flux computed, none of which needs the dt,
+// so the dt's chain of dependent loads overlaps their memory latency.
+template <class M, bool RESID, class DtOf>
 __device__ __forceinline__ void godunov_item(const HfStepArgs<typename M::R>& a, const HfItem it, const int lane,
-                                             const double dt_step, unsigned long long* __restrict__ imax_out)
+                                             DtOf&& dt_of, unsigned long long* __restrict__ imax_out)
 {
     typedef typename M::R R;
 
@@ -822,20 +825,18 @@ __device__ __forceinline__ void godunov_item(const HfStepArgs<typename M::R>& a,
 
     const R rdx = R(1.0 / bd.dx);                  // godunov.cpp:97-98
     const R rdy = R(1.0 / bd.dy);
-    const R dtR = R(dt_step);
     const R gam = R(a.gamma), g1 = R(a.gamma - 1.0);
 
     const int y0 = it.y0;
-    R ub[4], uc[4];
+    R ub[4], uc[4], un[4];
     ROW(y0 - 1, ub);
     ROW(y0, uc);
+    ROW(y0 + 1, un);                               // loaded one iteration ahead
     State<R> sb = make_state<M>(ub, gam, g1);
     State<R> sc = make_state<M>(uc, gam, g1);
     R Gs[4];
     hllc<M, false>(sb, sc, Gs);                    // face y0-1/2
-
-    R un[4];
-    ROW(y0 + 1, un);                               // loaded one iteration ahead
+    const R dtR = R(dt_of());
     R* Ob = RESID ? a.O : a.O + bd.off;
     double mx = 0.0, my = 0.0;
     for (int r = 0; r < it.rows; r++) {
@@ -882,18 +883,18 @@ __global__ __launch_bounds__(256) void k_godunov(const HfStepArgs<typename M::R>
     const unsigned n = (unsigned)a.nitems;
     for (unsigned idx = ticket_first(n); idx < n; idx = ticket_next(a.sync, lane, n)) {
         const HfItem it = a.items[a.item0 + idx];
-        double dt_step, dt_pred;
-        if (a.use_fixed) {
-            dt_step = a.fixed_step;
-        } else {
+        auto dt_of = [&]() -> double {
+            if (a.use_fixed) return a.fixed_step;
             const HfBlockDev bd = a.blocks[it.block];
+            double dt_step, dt_pred;
             long long kstep, kpred;
             step_dts<R>(a, bd, false, dt_step, dt_pred, kstep, kpred);
             const long long hs = kstep - a.hist_base;
             if (bd.problem == 0 && bd.lb == 0 && it.x0 == 0 && it.y0 == bd.g && hs >= 1 && hs <= a.hist_cap)
                 a.hist[hs - 1] = dt_step;
-        }
-        godunov_item<M, RESID>(a, it, lane, dt_step, a.imax_new);
+            return dt_step;
+        };
+        godunov_item<M, RESID>(a, it, lane, dt_of, a.imax_new);
     }
 }
 
