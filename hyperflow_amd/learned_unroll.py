@@ -37,7 +37,6 @@ sees exactly the cells whose domain of dependence lies in it.
 from __future__ import annotations
 
 import argparse
-import json
 import math
 import os
 import sys
