@@ -26,13 +26,18 @@ muscl_hancock.cpp:243-376) to rounding.  Its Van Leer and MinBee limiters
 follow vanleer.cpp:29-36 and minbee.cpp:29-38 with the ratio's tolerance
 clamp of muscl_hancock.cpp:78-94, its Riemann solver riemann_hllc.cpp:25-212.
 
-Crops.  A sample is a square crop of the 1024^2 coarse grid.  A step needs two
-cells of neighbours on every side and each 3x3 convolution one, so the state
-shrinks by 2 cells per side per step and 5 per correction ("valid"
-evaluation); no boundary condition is needed inside a crop, and the loss
-sees exactly the cells whose domain of dependence lies in it.
+Samples.  With --whole (how the shipped network was trained) a sample is a
+whole 1024^2 domain: the steps take the solver's transmissive ghosts and the
+network the zero padding hf_correct gives it at the domain edges, so training
+sees every cell as runs do.  Otherwise a sample is a square crop: a step needs
+two cells of neighbours on every side and each 3x3 convolution one, so the
+state shrinks by 2 cells per side per step and 5 per correction ("valid"
+evaluation) and the loss sees the cells whose domain of dependence lies in
+the crop; crops never train the cells along the domain edges, and what they
+learned did not carry over to runs.
 
-    python -m hyperflow_amd.learned_unroll train --init hyperflow_amd/learned_weights.npz --out w.npz
+    python -m hyperflow_amd.learned_unroll train --init hyperflow_amd/learned_weights.npz --out w.npz \
+        --whole --every 16 --unroll 1 --lr 3e-4 --seconds 420
 """
 from __future__ import annotations
 
