@@ -8,8 +8,8 @@
 // from HBM (plus the 2-lane halo at each strip edge and 2 halo rows per
 // strip) and every output written once; nothing intermediate goes to memory.
 // The new state's max wave speeds (for the next dt) are reduced in the same
-// pass: wave shuffles, then a 64-bit atomic max per wave whose value raises
-// the block's slot (raise_max).
+// pass: wave shuffles, then a 64-bit atomic max per wave (store_maxima; where
+// the host enables the probe, only from waves whose value raises the slot).
 //
 // The interior is read from A and the filled ghosts from G (the ghost fill
 // writes them into the destination buffer), so no launch ever reads a cell
@@ -88,11 +88,11 @@ __device__ __forceinline__ unsigned ticket_next(uint32_t* sync, int lane, unsign
 
 // What waves of one launch tell each other travels only through device-scope
 // atomics (wave-speed maxima, fixed flags, the pending bitmap, the counters):
-// writers update with atomic read-modify-writes (a maxima writer skips one
-// that a coherent load shows cannot raise the slot), and readers of values that
-// decide results read with coherent_read (hf_physics.h), an atomic
-// read-modify-write that changes nothing.  The new states themselves are not
-// read inside the launch.
+// writers update with atomic read-modify-writes (with the maxima probe, a
+// writer skips one that a coherent load shows cannot raise the slot), and
+// readers of values that decide results read with coherent_read
+// (hf_physics.h), an atomic read-modify-write that changes nothing.  The new
+// states themselves are not read inside the launch.
 
 // One item of the launch is complete.  This wave's atomics (its maxima and
 // fixed flag) must be performed before the count that waiting waves poll:
