@@ -56,3 +56,21 @@ def test_valu_bound_of_the_fp64_muscl_loop():
     assert 23300 < v["bound"] < 23450
     assert abs(v["frac"] - 18000.0 / v["bound"]) < 1e-4
     assert bench.valu_bound("Godunov", None, "fp64", 1e9) is None
+
+
+def test_learned_check_seeds_took_no_part_in_training_or_choice():
+    """The headline line's learned sub-object runs on seeds that neither the
+    training (seeds 0..39, the held-out ones included) nor the choice of the
+    network and its settings used."""
+    import numpy as np
+    z = np.load(os.path.join(ROOT, "hyperflow_amd", "learned_weights.npz"), allow_pickle=False)
+    meta = json.loads(str(z["meta"]))
+    lo, hi = meta["train_seeds"]
+    used = set(range(lo, hi + 1)) | set(meta["holdout_seeds"])
+    for stage in meta.get("unrolled", []):
+        a, b = stage["train_seeds"]
+        used |= set(range(a, b + 1)) | set(stage["holdout_seeds"])
+    check, choice = set(bench.LEARNED_SEEDS_CHECK), set(bench.LEARNED_SEEDS_CHOICE)
+    assert check and not check & choice
+    assert not (check | choice) & used
+    assert meta["every"] == meta["trained_every"]      # runs correct at the interval trained for
