@@ -904,8 +904,9 @@ __global__ __launch_bounds__(256) void k_godunov(const HfStepArgs<typename M::R>
 // ghost cell, which pre-fill cell (or Dirichlet constant) it ends up holding
 // and which momentum components were negated on the way.  Reads come from S,
 // writes go to D, so the fill is race-free in one launch.  Also reduces the
-// ghosts' max wave speeds per block, and (step mode) advances each problem's
-// TimeStep call counter and clears the partial maxima of the next step.
+// ghosts' max wave speeds per block, and (step mode, in one extra workgroup
+// after those of the ghost cells) advances each problem's TimeStep call
+// counter and clears the partial maxima of the next step.
 
 constexpr int GHOST_WAVES = 4;   // k_ghost workgroups: 256 threads
 template <class M>
@@ -916,7 +917,9 @@ __global__ __launch_bounds__(256) void k_ghost(const HfGhost* __restrict__ tab, 
                                                double gamma, GhostBook book)
 {
     typedef typename M::R R;
-    if (blockIdx.x == 0) {
+    // The last workgroup keeps the step's books (none of it reads or writes a
+    // ghost cell or gmax), beside the ghost cells instead of ahead of some.
+    if (blockIdx.x == gridDim.x - 1) {
         for (int p = threadIdx.x; p < book.nprob && book.bump; p += blockDim.x)
             book.calls[p] += (uint32_t)book.probs[p].cps;
         for (int i = threadIdx.x; i < book.nclear; i += blockDim.x) {
@@ -948,6 +951,7 @@ __global__ __launch_bounds__(256) void k_ghost(const HfGhost* __restrict__ tab, 
                 book.imax_fix[2 * (t * nb + b) + 1] = 0ull;
             }
         }
+        return;
     }
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;

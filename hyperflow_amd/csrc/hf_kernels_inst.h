@@ -66,7 +66,7 @@ hipError_t launch_ghost<Real>(const HfGhost* tab, int n, const HfBlockDev* block
                               hipStream_t s)
 {
     const int threads = 64 * GHOST_WAVES;
-    const int nb = n > 0 ? (n + threads - 1) / threads : 1;
+    const int nb = (n > 0 ? (n + threads - 1) / threads : 0) + 1;   // + the bookkeeping workgroup
     hipLaunchKernelGGL((k_ghost<Math>), dim3(nb), dim3(threads), 0, s, tab, n, blocks, S, D, consts, gmax, gamma,
                        book);
     return hipGetLastError();
